@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X H-apply path (one GPU).
+
+Default workload (the configuration the headline metric is quoted on): complex FP64
+H = -1/2 Laplacian + v_loc + V_nl on the sphere R = 12 bohr, h = 0.25 (462,781 points),
+order-4 star stencil, 64 states in two packed batches of 32, 16 Gaussian wells,
+16 atoms x 4 Kleinman-Bylander projectors.  One step = H applied to both batches.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor]
+
+Prints one JSON line.  `value` is state-points/s with the states already resident in
+HBM; `roofline` is the dominant kernel (k_apply, the fused stencil + local + ket pass)
+timed with HIP events on its launch stream over the timed region; `cpu_baseline` is the
+CPU oracle (C/OpenMP restatement of the reference CPU path) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "octopus-debian-package_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 H-apply state-points/s & %HBM peak; batched vs plain; adjoint step time"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def hamiltonian_bytes(s, nst: int, complex_: bool = True) -> int:
+    """Compulsory HBM bytes of one H-apply of a batch: psi read over np_part rows, H psi
+    written over np rows, v, the projector matrices, maps and their points, weights."""
+    es = 16 if complex_ else 8
+    b = es * nst * (s.np_part + s.np) + 8 * s.np + 8 * 25
+    if s.proj is not None:
+        o = s.proj.offsets
+        b += int(sum(8 * int(npt) * int(npj) + 4 * int(npt) + 8 * int(npt) for npt, npj in zip(o[:, 0], o[:, 1])))
+    return b
+
+
+def cpu_baseline_hamiltonian(s, nst: int, budget_s: float) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+    from octhip import workloads
+    z = workloads.gaussian_envelope_states(s, nst)
+    psi = workloads.pack(z, s.np_part)
+    h = O.Hamiltonian(s.op, s.v, s.proj)
+    h.apply(psi)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        h.apply(psi)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 50:
+            break
+    per = el / reps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": s.np * nst / per, "unit": "state-points/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} H-applies of one {nst}-state complex batch on the full R=12 sphere "
+                      f"({per * 1e3:.1f} ms each, OpenMP oracle)"}
+
+
+def bench_hamiltonian(args, torch, api, _lib, workloads):
+    log("building the R=12 h=0.25 sphere workload")
+    s = workloads.hamiltonian_system()
+    nst_total, pack = 64, 32
+    hm = api.HamiltonianDevice.from_system(s)
+    if args.xcd_chunk:
+        hm.kinetic.set_schedule(args.xcd_chunk)
+    hm.set_bra_mode(args.bra)
+    batches, outs = [], []
+    for b in range(nst_total // pack):
+        z = workloads.gaussian_envelope_states(s, pack, first=b * pack)
+        batches.append(api.Batch.from_states(z, s.np_part))
+        outs.append(api.Batch(pack, s.np, True))
+    torch.cuda.synchronize()
+
+    def step():
+        for pb, hb in zip(batches, outs):
+            api.hamiltonian_elec_apply_batch(hm, pb, hb)
+
+    log(f"np={s.np} np_part={s.np_part} warmup={args.warmup} steps={args.steps}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.profiling_start()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.profiling_stop()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern = {}
+    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_zero_rows"):
+        tot, n = _lib.profiling_get(k)
+        kern[k] = {"avg_us": tot * 1e3 / n if n else None, "launches": n}
+    bytes_batch = hamiltonian_bytes(s, pack)
+    t_apply = kern["k_apply"]["avg_us"] * 1e-6
+    achieved = bytes_batch / t_apply / 1e9
+    value = s.np * nst_total / (ms_step * 1e-3)
+    happly_gbs = bytes_batch * (nst_total // pack) / (ms_step * 1e-3) / 1e9
+    res = {
+        "metric": METRIC, "value": value, "unit": "state-points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian-envelope states, Gaussian wells, synthetic KB projectors)",
+        "config": {"workload": "complex H-apply, sphere R=12 h=0.25 (np=462781, np_part=564189), "
+                               "order-4 star, 64 states as 2 packed batches of 32, 16 wells, "
+                               "16 atoms x 4 KB projectors (rc=3)",
+                   "states": nst_total, "pack_size": pack, "np": s.np, "np_part": s.np_part},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "kernel": "k_apply", "algorithmic_bytes_per_launch": bytes_batch,
+                     "avg_launch_us": kern["k_apply"]["avg_us"]},
+        "h_apply_hbm_frac": happly_gbs / HBM_PEAK_GBS,
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle on one batch")
+        res["cpu_baseline"] = cpu_baseline_hamiltonian(s, pack, args.cpu_seconds)
+    return res
+
+
+def bench_laplacian(args, torch, api, _lib, workloads):
+    from octhip.stencil import star_points
+    log("building the 64^3 cube workload")
+    s = workloads.laplacian_system()
+    f = workloads.gaussian_functions(s, 8)
+    fi = api.Batch.from_states(f, s.np_part, 8)
+    fo = api.Batch(8, s.np, False, 8)
+    op = api.NLOperatorDevice(s.op, star_points(4))
+    if args.xcd_chunk:
+        op.set_schedule(args.xcd_chunk)
+    flush = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB, evicts the MALL
+    for _ in range(args.warmup):
+        api.nl_operator_operate_batch(op, fi, fo)
+    torch.cuda.synchronize()
+    _lib.profiling_start()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        flush.zero_()
+        api.nl_operator_operate_batch(op, fi, fo)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.profiling_stop()
+    tot, n = _lib.profiling_get("k_apply")
+    t_k = tot * 1e-3 / n
+    b = 8 * 8 * (s.np_part + s.np) + 8 * 25
+    achieved = b / t_k / 1e9
+    return {"metric": "FP64 Laplacian state-points/s", "value": s.np * 8 / t_k, "unit": "state-points/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "real Laplacian, 64^3 cube h=0.3, 8 functions, pack 8, "
+                                                        "MALL flushed between reps"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_apply",
+                         "algorithmic_bytes_per_launch": b, "avg_launch_us": t_k * 1e6}}
+
+
+def bench_taylor(args, torch, api, _lib, workloads):
+    log("building the R=12 h=0.25 sphere workload (Taylor)")
+    s = workloads.hamiltonian_system()
+    hm = api.HamiltonianDevice.from_system(s)
+    if args.xcd_chunk:
+        hm.kinetic.set_schedule(args.xcd_chunk)
+    batches = [api.Batch.from_states(workloads.gaussian_envelope_states(s, 32, first=32 * b), s.np_part)
+               for b in range(4)]
+    te = api.Exponential(4)
+    dt = args.dt
+    for _ in range(args.warmup):
+        for pb in batches:
+            te.apply_batch(hm, pb, dt)
+    torch.cuda.synchronize()
+    _lib.profiling_start()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for pb in batches:
+            te.apply_batch(hm, pb, dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.profiling_stop()
+    ms = (t1 - t0) * 1e3 / args.steps
+    kern = {k: _lib.profiling_get(k) for k in ("k_apply", "k_bra", "k_bra_reduce", "k_zero_rows")}
+    return {"metric": "FP64 Taylor-4 step time (128 states)", "value": s.np * 128 * 4 / (ms * 1e-3),
+            "unit": "state-point H-applies/s", "ms_per_step": ms, "steps": args.steps, "warmup": args.warmup,
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Taylor-4 dt={dt}, 128 states as 4 batches of 32, R=12 sphere"},
+            "kernels": {k: {"avg_us": v[0] * 1e3 / v[1] if v[1] else None, "launches": v[1]}
+                        for k, v in kern.items()}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor"])
+    ap.add_argument("--xcd-chunk", type=int, default=0)
+    ap.add_argument("--bra", type=int, default=1)
+    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per k_apply launch measured with rocprofv3 (FETCH+WRITE, scaled)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(the path shards by state batches/systems without communication; see the design notes)",
+              file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    from octhip import _lib, api, workloads
+    _lib.init(0)
+    fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor}[args.workload]
+    res = fn(args, torch, api, _lib, workloads)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,182 @@
+/*
+ * octhip.h -- C ABI of the MI355X (gfx950) real-space H-apply path.
+ *
+ * Drop-in device backend for three Octopus routines, keeping Octopus's packed batch
+ * layout (batch_t ff_pack: rows = mesh points, 2^ldf contiguous doubles per row, state
+ * index fastest, complex states as interleaved re/im doubles):
+ *
+ *   nl_operator_operate_batch      src/grid/nl_operator_inc.F90:21-28 (device branch
+ *                                  operate_opencl, 257-469; CPU leaf
+ *                                  doperate_ri_vec, src/grid/operate.c:40-64)
+ *   hamiltonian_elec_apply_batch   src/hamiltonian/hamiltonian_elec_inc.F90:77-261
+ *   exponential_apply_batch        src/electrons/exponential.F90:581-757 (Taylor)
+ *
+ * Conventions shared by every entry point:
+ *   - every call returns an int status: OCTHIP_SUCCESS (0) or a negative OCTHIP_ERR_*;
+ *     octhip_get_last_error() copies the message of the last failure on this thread.
+ *     Octopus's binding turns a nonzero status into messages_fatal, as accel.F90 does
+ *     for CUDA errors.
+ *   - state buffers (fi, fo, psi, hpsi) are DEVICE pointers owned by the caller
+ *     (batch_t%ff_device); descriptors passed to *_create are HOST arrays that the
+ *     library deep-copies to the device (as nl_operator_build / build_accel upload).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  Calls are
+ *     asynchronous on that stream; octhip_stream_synchronize() mirrors accel_finish.
+ *   - an input batch has at least np_part rows, an output batch at least np rows,
+ *     both with row length 2^ldf doubles (ldf = log2(pack_size_real(1))).
+ *   - call from one host thread per handle; handles are not thread safe.
+ */
+#ifndef OCTHIP_H
+#define OCTHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define OCTHIP_SUCCESS           0
+#define OCTHIP_ERR_INVALID      -1   /* bad argument / descriptor */
+#define OCTHIP_ERR_HIP          -2   /* HIP runtime error */
+#define OCTHIP_ERR_ALLOC        -3   /* device or host allocation failed */
+#define OCTHIP_ERR_UNSUPPORTED  -4   /* valid input the device path does not cover */
+#define OCTHIP_ERR_STATE        -5   /* library not initialised / wrong handle state */
+
+/* points selector, values of src/grid/nl_operator.F90:130 */
+#define OCTHIP_OP_INNER 1
+#define OCTHIP_OP_OUTER 2
+#define OCTHIP_OP_ALL   3
+
+/* Hamiltonian terms, values of src/hamiltonian/hamiltonian_elec_base.F90:150-155 */
+#define OCTHIP_TERM_KINETIC            1
+#define OCTHIP_TERM_LOCAL_POTENTIAL    2
+#define OCTHIP_TERM_NON_LOCAL_POTENTIAL 4
+#define OCTHIP_TERM_ALL                0x7fffffff
+
+typedef struct octhip_nl_operator_s* octhip_nl_operator_t;
+typedef struct octhip_hamiltonian_s* octhip_hamiltonian_t;
+
+/* ---------------------------------------------------------------- lifecycle --
+ * Replaces accel_init / accel_end (src/basic/accel.F90:264-700, 1105-1175). */
+int octhip_init(int device);
+int octhip_finalize(void);
+int octhip_get_last_error(char* buf, size_t len);
+int octhip_version(int* major, int* minor);
+
+/* ------------------------------------------------------------------- memory --
+ * Backing store of accel_mem_t%mem (src/basic/accel.F90:150-161, accel_create_buffer,
+ * accel_write_buffer, accel_read_buffer, accel_release_buffer, accel_finish). */
+int octhip_malloc(void** ptr, size_t bytes);
+int octhip_free(void* ptr);
+int octhip_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int octhip_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int octhip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int octhip_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------- nl_operator --
+ * octhip_nl_operator_create replaces the device half of nl_operator_build
+ * (src/grid/nl_operator.F90:516-582: buff_ri, buff_map, kernel build).  Inputs are the
+ * host arrays nl_operator_build produced, with the C leaf's conventions
+ * (src/grid/operate.c:40-45): n = stencil size, w = const weights op%w(:,1),
+ * ri = op%ri as n*nri ints (offset of neighbour j of run l at ri[l*n+j]),
+ * rimap_inv = op%rimap_inv (nri+1 ints: run l covers points [rimap_inv[l],
+ * rimap_inv[l+1]) 0-based).  stencil_points = op%stencil%points (3*n ints, x,y,z of
+ * point j at [3*j..3*j+2]) lets the library rebuild the lattice and index neighbours
+ * through a compact site map instead of ri; pass NULL to index through ri.
+ * coords (3*np_part ints, optional) are the lattice coordinates of every point. */
+int octhip_nl_operator_create(int n, const double* w, const int* stencil_points,
+                              int nri, const int* ri, const int* rimap_inv,
+                              int np, int np_part, const int* coords,
+                              octhip_nl_operator_t* op);
+int octhip_nl_operator_destroy(octhip_nl_operator_t op);
+/* 1 if neighbours are indexed through the rebuilt lattice, 0 if through ri */
+int octhip_nl_operator_uses_lattice(octhip_nl_operator_t op, int* flag);
+
+/* fo(:, 0:np-1) = sum_j (factor*w[j]) * fi(:, p + ri[rimap(p)][j]).
+ * Mirrors X(nl_operator_operate_batch)(op, fi, fo, points=points, factor=factor) on a
+ * BATCH_DEVICE_PACKED batch (src/grid/nl_operator_inc.F90:72-101); the per-lane sum runs
+ * over j in stencil order with fused multiply-adds, like the C leaf
+ * (src/grid/operate_inc.c:76-118).  Boundary rows (np..np_part-1) of fi are READ as
+ * they are: the caller sets them (boundaries_set), as Octopus does before operate.
+ * Only points == OCTHIP_OP_ALL is supported (inner/outer splits exist only with domain
+ * parallelisation). */
+int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const double* fi, double* fo,
+                                     int ldf, double factor, int points, void* stream);
+
+/* Same contract as the CPU leaf doperate_ri_vec/zoperate_ri_vec
+ * (src/grid/operate.c:40-76), on device memory: ri, rimap_inv, rimap_inv_max, fi, fo
+ * are device pointers, w is a host array of n doubles.  Runs l = 0..nri-1 cover
+ * points [rimap_inv[l], rimap_inv_max[l]).  Stateless, for callers that keep Octopus's
+ * own index buffers on the device. */
+int octhip_operate_ri_vec(int n, const double* w, int nri, const int* ri,
+                          const int* rimap_inv, const int* rimap_inv_max,
+                          const double* fi, int ldf, double* fo, void* stream);
+
+/* ------------------------------------------------------------- hamiltonian --
+ * octhip_hamiltonian_create replaces hamiltonian_elec_base_build_proj's device upload
+ * (src/hamiltonian/hamiltonian_elec_base.F90:755-909).  kinetic: the Laplacian operator
+ * (hm%hm_base%kinetic); mass: hm%mass (kinetic factor -1/(2 mass),
+ * src/hamiltonian/hamiltonian_elec_inc.F90:155); vlocal: np doubles (total local
+ * potential, src/hamiltonian/hamiltonian_elec.F90:1323); projector matrices in
+ * build_accel's flat layout: offsets = 6*nmat ints (POINTS, PROJS, MATRIX, MAP, SCAL,
+ * MIX per matrix; MIX must be -1: KB projectors carry no mixing matrix), matrices =
+ * concatenated column-major (npoints x nprojs) blocks, maps = concatenated 1-based
+ * mesh indices (inner points), scals = concatenated scal(:).  nmat may be 0. */
+int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double mass, const double* vlocal,
+                              int nmat, const int* offsets, const double* matrices,
+                              const int* maps, const double* scals,
+                              octhip_hamiltonian_t* hm);
+int octhip_hamiltonian_destroy(octhip_hamiltonian_t hm);
+/* replace the local potential (np doubles, host) -- hamiltonian_elec_update */
+int octhip_hamiltonian_set_potential(octhip_hamiltonian_t hm, const double* vlocal, void* stream);
+/* v_eff(p) = vlocal(p) + sum_d field[d] * x_d(p): the length-gauge dipole term of
+ * laser_potential (src/hamiltonian/lasers.F90:823-853, added with a + sign).
+ * positions = 3*np doubles (x,y,z of point p at [3p..3p+2]); field = 3 doubles.
+ * Positions are uploaded once; later calls with positions == NULL only change field. */
+int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t hm, const double* positions,
+                                        const double* field, void* stream);
+
+/* hpsi = H psi on a packed batch: X(hamiltonian_elec_apply_batch)(hm, mesh, psib, hpsib,
+ * terms, set_bc) (src/hamiltonian/hamiltonian_elec_inc.F90:77-261) for the terms
+ * kinetic (-1/2m Laplacian), local potential and KB nonlocal projectors, in Octopus's
+ * order: hpsi = K psi; hpsi += v psi; hpsi += sum_a P_a diag(scal_a) P_a^T psi.
+ * psi: device, >= np_part rows; hpsi: device, >= np rows; 2^ldf doubles per row.
+ * set_bc != 0 zeroes psi's boundary rows first (boundaries_set, zero Dirichlet,
+ * src/grid/boundaries_inc.F90:275-363); with set_bc == 0 boundary rows are read as
+ * they are.  is_complex: batch holds complex states (lanes are re/im pairs). */
+int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t hm, double* psi, double* hpsi,
+                                   int ldf, int is_complex, int terms, int set_bc,
+                                   void* stream);
+
+/* ----------------------------------------------------------------- taylor --
+ * psi <- sum_{k=0..order} (-i deltat H)^k / k! psi in place: exponential_apply_batch
+ * with TDExponentialMethod = taylor (src/electrons/exponential.F90:581-757).
+ * imag_time != 0 uses deltat -> i*deltat (exponential.F90:609-619).  The batch must be
+ * complex (ldf >= 1).  psi's boundary rows are zeroed (boundaries_set).  Work buffers
+ * are owned by the handle and reused across calls. */
+int octhip_exponential_taylor_batch(octhip_hamiltonian_t hm, double* psi, int ldf,
+                                    double deltat, int order, int imag_time, void* stream);
+
+/* --------------------------------------------------------------- profiling --
+ * Per-kernel device timing with HIP events recorded on the launch stream, the device-side
+ * counterpart of profiling_in/profiling_out (src/basic/profiling.F90:503-625).  Regions
+ * are kernel names ("k_apply", "k_bra", "k_bra_reduce", "k_zero_rows", "k_zaxpy").
+ * start clears earlier records; get synchronises the recorded events and returns the
+ * summed milliseconds and the number of launches of one region. */
+int octhip_profiling_start(void);
+int octhip_profiling_stop(void);
+int octhip_profiling_get(const char* region, double* total_ms, int* count);
+
+/* ------------------------------------------------------------------ tuning --
+ * No counterpart in the reference (Octopus fixes launch shapes in accel.F90).  Used by the
+ * autotuner; results never depend on these settings beyond summation order.
+ * xcd_chunk: consecutive point tiles given to one XCD before moving to the next
+ * (1 = hardware round-robin).  bra_mode: 0 = plain FMA reduction, 1 = FP64 MFMA. */
+int octhip_nl_operator_set_schedule(octhip_nl_operator_t op, int xcd_chunk);
+int octhip_hamiltonian_set_bra_mode(octhip_hamiltonian_t hm, int bra_mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTHIP_H */

@@ -1,0 +1,365 @@
+// Hamiltonian handle: local potential, KB projectors, fused apply, Taylor propagator.
+#include <complex>
+#include <cstring>
+
+#include "kernels.h"
+
+using namespace octhip;
+
+namespace octhip {
+void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a);
+int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s);
+}  // namespace octhip
+
+namespace {
+
+enum { O_POINTS = 0, O_PROJS, O_MATRIX, O_MAP, O_SCAL, O_MIX };
+
+int ensure_work(octhip_hamiltonian_s* h, int ldf) {
+  if (h->work_ldf >= ldf) return OCTHIP_SUCCESS;
+  dfree(h->d_partial);
+  dfree(h->d_proj);
+  h->d_partial = h->d_proj = nullptr;
+  const size_t L = (size_t)1 << ldf;
+  int rc = OCTHIP_SUCCESS;
+  if (h->nmat > 0) {
+    rc = dalloc((void**)&h->d_partial, sizeof(double) * h->nchunk * h->max_nprojs * L);
+    if (!rc) rc = dalloc((void**)&h->d_proj, sizeof(double) * h->full_projection_size * L);
+  }
+  if (rc) return rc;
+  h->work_ldf = ldf;
+  return OCTHIP_SUCCESS;
+}
+
+int ensure_taylor(octhip_hamiltonian_s* h, int ldf) {
+  if (h->y_ldf >= ldf) return OCTHIP_SUCCESS;
+  for (auto& y : h->d_y) {
+    dfree(y);
+    y = nullptr;
+  }
+  const size_t bytes = sizeof(double) * ((size_t)h->np << ldf);
+  for (auto& y : h->d_y) {
+    int rc = dalloc((void**)&y, bytes);
+    if (rc) return rc;
+  }
+  h->y_ldf = ldf;
+  return OCTHIP_SUCCESS;
+}
+
+// projections of batch f (L lanes) into h->d_proj
+int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s) {
+  if (h->nmat == 0) return OCTHIP_SUCCESS;
+  BraArgs b;
+  b.psi = f;
+  b.L = L;
+  b.chunks = h->d_chunks;
+  b.mat_info = h->d_mat_info;
+  b.mats = h->d_matrices;
+  b.maps = h->d_maps;
+  b.partial = h->d_partial;
+  b.pjmax = h->max_nprojs;
+  int rc = (h->use_mfma && L >= 16) ? launch_bra_mfma(b, h->d_pt, h->pjs, h->nchunk, s)
+                                    : launch_bra(b, h->nchunk, false, s);
+  if (rc) return rc;
+  BraReduceArgs r;
+  r.partial = h->d_partial;
+  r.proj_mat = h->d_proj_mat;
+  r.mat_pbase = h->d_mat_pbase;
+  r.mat_chunk0 = h->d_mat_chunk0;
+  r.scals = h->d_scals;
+  r.proj = h->d_proj;
+  r.L = L;
+  r.pjmax = h->max_nprojs;
+  r.nproj = h->full_projection_size;
+  return launch_bra_reduce(r, s);
+}
+
+int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s) {
+  const size_t count = (size_t)(np_part - np) * L;
+  if (count == 0) return OCTHIP_SUCCESS;
+  size_t grid = (count + BLOCK - 1) / BLOCK;
+  if (grid > 8192) grid = 8192;
+  ProfScope prof("k_zero_rows", s);
+  hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)grid), dim3(BLOCK), 0, s, f, (size_t)np * L, count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zero_rows: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+void fill_terms(const octhip_hamiltonian_s* h, int terms, ApplyArgs& a) {
+  if (terms & OCTHIP_TERM_KINETIC) {
+    stencil_args(h->kin, -0.5 / h->mass, a);  // factor -1/(2m), hamiltonian_elec_inc.F90:155
+  } else {
+    stencil_args(h->kin, 0.0, a);
+    a.n = 0;  // no stencil term: runtime-size path with zero points
+  }
+  if ((terms & OCTHIP_TERM_LOCAL_POTENTIAL) && h->d_v) {
+    a.v = h->d_v;
+    if (h->has_field && h->d_xyz) {
+      a.xyz = h->d_xyz;
+      a.has_field = 1;
+      a.f0 = h->field[0];
+      a.f1 = h->field[1];
+      a.f2 = h->field[2];
+    }
+  }
+  if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
+    a.pos = h->d_pos;
+    a.ent = h->d_ent;
+    a.pt = h->d_pt;
+    a.mat_pbase = h->d_mat_pbase;
+    a.mat_npj = h->d_mat_npj;
+    a.pjs = h->pjs;
+    a.proj = h->d_proj;
+  }
+}
+
+}  // namespace
+
+extern "C" int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double mass, const double* vlocal,
+                                         int nmat, const int* offsets, const double* matrices,
+                                         const int* maps, const double* scals, octhip_hamiltonian_t* out) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(out != nullptr, "hamiltonian_create: null output handle");
+  *out = nullptr;
+  OCTHIP_REQUIRE(kinetic != nullptr, "hamiltonian_create: null kinetic operator");
+  OCTHIP_REQUIRE(mass > 0.0, "mass must be positive");
+  OCTHIP_REQUIRE(nmat >= 0, "nmat < 0");
+  OCTHIP_REQUIRE(nmat == 0 || (offsets && matrices && maps && scals), "null projector arrays");
+  const int np = kinetic->np;
+  // sizes and validation of build_accel's offsets
+  size_t matrix_size = 0, total_points = 0, nproj = 0;
+  int max_nprojs = 0;
+  for (int a = 0; a < nmat; ++a) {
+    const int* o = offsets + 6 * (size_t)a;
+    OCTHIP_REQUIRE(o[O_POINTS] >= 0 && o[O_PROJS] >= 1, "matrix %d: bad POINTS/PROJS", a);
+    OCTHIP_REQUIRE(o[O_MIX] == -1, "matrix %d: mixing matrices (HGH/SOC) are not supported", a);
+    OCTHIP_REQUIRE(o[O_MATRIX] >= 0 && o[O_MAP] >= 0 && o[O_SCAL] >= 0, "matrix %d: negative offset", a);
+    matrix_size = std::max(matrix_size, (size_t)o[O_MATRIX] + (size_t)o[O_POINTS] * o[O_PROJS]);
+    total_points = std::max(total_points, (size_t)o[O_MAP] + o[O_POINTS]);
+    nproj = std::max(nproj, (size_t)o[O_SCAL] + o[O_PROJS]);
+    max_nprojs = std::max(max_nprojs, o[O_PROJS]);
+  }
+  for (size_t i = 0; i < total_points; ++i)
+    OCTHIP_REQUIRE(maps[i] >= 1 && maps[i] <= np, "map entry %zu = %d outside 1..np", i, maps[i]);
+  auto* h = new octhip_hamiltonian_s();
+  h->kin = kinetic;
+  h->mass = mass;
+  h->np = np;
+  h->nmat = nmat;
+  h->max_nprojs = max_nprojs;
+  h->full_projection_size = (int)nproj;
+  h->total_points = (int)total_points;
+  h->pjs = max_nprojs;
+  h->offsets.assign(offsets ? offsets : nullptr, offsets ? offsets + 6 * (size_t)nmat : nullptr);
+  hipStream_t s = nullptr;
+  int rc = OCTHIP_SUCCESS;
+  if (vlocal) rc = upload(&h->d_v, vlocal, (size_t)np, s);
+  if (!rc && nmat > 0) {
+    std::vector<double> pt(total_points * (size_t)h->pjs, 0.0);
+    std::vector<int> cnt(np + 1, 0), pbase(nmat), npj(nmat), proj_mat(nproj, 0), chunk0(nmat + 1);
+    std::vector<int4> info(nmat), chunks;
+    for (int a = 0; a < nmat; ++a) {
+      const int* o = offsets + 6 * (size_t)a;
+      const int npt = o[O_POINTS], nj = o[O_PROJS];
+      pbase[a] = o[O_SCAL];
+      npj[a] = nj;
+      info[a] = make_int4(npt, nj, o[O_MATRIX], o[O_MAP]);
+      for (int j = 0; j < nj; ++j) proj_mat[o[O_SCAL] + j] = a;
+      for (int q = 0; q < npt; ++q) {
+        for (int j = 0; j < nj; ++j)
+          pt[(size_t)(o[O_MAP] + q) * h->pjs + j] = matrices[o[O_MATRIX] + (size_t)j * npt + q];
+        cnt[maps[o[O_MAP] + q]]++;  // 1-based -> counts at index p+1
+      }
+      chunk0[a] = (int)chunks.size();
+      for (int q0 = 0; q0 < npt; q0 += BRA_CHUNK)
+        chunks.push_back(make_int4(a, q0, std::min(q0 + BRA_CHUNK, npt), (int)chunks.size() - chunk0[a]));
+    }
+    chunk0[nmat] = (int)chunks.size();
+    // CSR of (sphere point, matrix) per mesh point, matrix order preserved
+    std::vector<int> pos(np + 1, 0);
+    for (int p = 0; p < np; ++p) pos[p + 1] = pos[p] + cnt[p + 1];
+    std::vector<int2> ent(pos[np]);
+    std::vector<int> fillp(pos.begin(), pos.end() - 1);
+    for (int a = 0; a < nmat; ++a) {
+      const int* o = offsets + 6 * (size_t)a;
+      for (int q = 0; q < o[O_POINTS]; ++q) {
+        const int p = maps[o[O_MAP] + q] - 1;
+        ent[fillp[p]++] = make_int2(o[O_MAP] + q, a);
+      }
+    }
+    h->nchunk = (int)chunks.size();
+    rc = upload(&h->d_matrices, matrices, matrix_size, s);
+    if (!rc) rc = upload(&h->d_maps, maps, total_points, s);
+    if (!rc) rc = upload(&h->d_scals, scals, nproj, s);
+    if (!rc) rc = upload(&h->d_pt, pt.data(), pt.size(), s);
+    if (!rc) rc = upload(&h->d_pos, pos.data(), pos.size(), s);
+    if (!rc) rc = upload(&h->d_ent, ent.data(), ent.size(), s);
+    if (!rc) rc = upload(&h->d_mat_pbase, pbase.data(), pbase.size(), s);
+    if (!rc) rc = upload(&h->d_mat_npj, npj.data(), npj.size(), s);
+    if (!rc) rc = upload(&h->d_chunks, chunks.data(), chunks.size(), s);
+    if (!rc) rc = upload(&h->d_mat_chunk0, chunk0.data(), chunk0.size(), s);
+    if (!rc) rc = upload(&h->d_mat_info, info.data(), info.size(), s);
+    if (!rc) rc = upload(&h->d_proj_mat, proj_mat.data(), proj_mat.size(), s);
+  }
+  if (rc) {
+    octhip_hamiltonian_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
+  if (!h) return OCTHIP_SUCCESS;
+  (void)hipDeviceSynchronize();
+  dfree(h->d_mat_info);
+  dfree(h->d_proj_mat);
+  dfree(h->d_v);
+  dfree(h->d_xyz);
+  dfree(h->d_matrices);
+  dfree(h->d_maps);
+  dfree(h->d_scals);
+  dfree(h->d_pt);
+  dfree(h->d_pos);
+  dfree(h->d_ent);
+  dfree(h->d_mat_pbase);
+  dfree(h->d_mat_npj);
+  dfree(h->d_chunks);
+  dfree(h->d_mat_chunk0);
+  dfree(h->d_partial);
+  dfree(h->d_proj);
+  dfree(h->d_y[0]);
+  dfree(h->d_y[1]);
+  delete h;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_set_bra_mode(octhip_hamiltonian_t h, int bra_mode) {
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  OCTHIP_REQUIRE(bra_mode == 0 || bra_mode == 1, "bra_mode must be 0 or 1");
+  h->use_mfma = bra_mode;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_set_potential(octhip_hamiltonian_t h, const double* vlocal, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && vlocal, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (!h->d_v) {
+    int rc = dalloc((void**)&h->d_v, sizeof(double) * h->np);
+    if (rc) return rc;
+  }
+  OCTHIP_HIP(hipMemcpyAsync(h->d_v, vlocal, sizeof(double) * h->np, hipMemcpyHostToDevice, s));
+  OCTHIP_HIP(hipStreamSynchronize(s));
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const double* positions,
+                                                   const double* field, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && field, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (positions) {
+    dfree(h->d_xyz);
+    h->d_xyz = nullptr;
+    int rc = upload(&h->d_xyz, positions, 3 * (size_t)h->np, s);
+    if (rc) return rc;
+  }
+  OCTHIP_REQUIRE(h->d_xyz != nullptr, "set_dipole_field: positions never uploaded");
+  for (int d = 0; d < 3; ++d) h->field[d] = field[d];
+  h->has_field = true;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
+                                              int is_complex, int terms, int set_bc, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  OCTHIP_REQUIRE(psi && hpsi && psi != hpsi, "bad batch pointers");
+  OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);
+  OCTHIP_REQUIRE(!is_complex || ldf >= 1, "complex batch needs ldf >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  const int L = 1 << ldf;
+  const octhip_nl_operator_s* op = h->kin;
+  int rc = ensure_work(h, ldf);
+  if (rc) return rc;
+  if (set_bc) {
+    rc = zero_boundary(psi, op->np, op->np_part, L, s);
+    if (rc) return rc;
+  }
+  if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
+    rc = run_bra(h, psi, L, s);
+    if (rc) return rc;
+  }
+  ApplyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  fill_terms(h, terms, a);
+  a.fi = psi;
+  a.fo = hpsi;
+  a.L = L;
+  a.lim = set_bc ? op->np : op->np_part;
+  return launch_apply(a, op->lattice && a.n > 0, EPI_STORE, s);
+}
+
+extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
+                                               int order, int imag_time, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && psi, "null argument");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  hipStream_t s = (hipStream_t)stream;
+  const int L = 1 << ldf;
+  const octhip_nl_operator_s* op = h->kin;
+  int rc = ensure_work(h, ldf);
+  if (!rc) rc = ensure_taylor(h, ldf);
+  if (!rc) rc = zero_boundary(psi, op->np, op->np_part, L, s);
+  if (rc) return rc;
+  // zfact *= (-i * deltat) / k  (exponential.F90:720-724), deltat -> i*deltat in imaginary time
+  const std::complex<double> dt_c = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
+  const std::complex<double> mi(-0.0, -1.0);
+  const std::complex<double> step(mi.real() * dt_c.real() - mi.imag() * dt_c.imag(),
+                                  mi.real() * dt_c.imag() + mi.imag() * dt_c.real());
+  std::complex<double> z(1.0, 0.0), z1;
+  for (int k = 1; k <= order; ++k) {
+    const std::complex<double> zz(z.real() * step.real() - z.imag() * step.imag(),
+                                  z.real() * step.imag() + z.imag() * step.real());
+    z = std::complex<double>(zz.real() / k, zz.imag() / k);
+    const double* yin = (k == 1) ? psi : h->d_y[(k - 2) & 1];
+    double* yout = (k < order) ? h->d_y[(k - 1) & 1] : nullptr;
+    rc = run_bra(h, yin, L, s);
+    if (rc) return rc;
+    ApplyArgs a;
+    std::memset(&a, 0, sizeof(a));
+    fill_terms(h, OCTHIP_TERM_ALL, a);
+    a.fi = yin;
+    a.L = L;
+    a.lim = op->np;  // boundary rows are zero: psi's were just cleared, work buffers have none
+    if (order == 1) {
+      a.fo = h->d_y[0];
+      rc = launch_apply(a, op->lattice, EPI_STORE, s);
+      if (rc) return rc;
+      const size_t count = (size_t)op->np * L;
+      size_t grid = std::min<size_t>((count + BLOCK - 1) / BLOCK, 8192);
+      ProfScope prof("k_zaxpy", s);
+      hipLaunchKernelGGL(k_zaxpy, dim3((unsigned)grid), dim3(BLOCK), 0, s, psi, h->d_y[0], count, z.real(), z.imag());
+      OCTHIP_HIP(hipGetLastError());
+      break;
+    }
+    a.yout = yout;
+    if (k == 1) {
+      a.acc = nullptr;
+      z1 = z;
+    } else {
+      a.acc = psi;
+      a.has_c1 = (k == 2);
+      a.c1r = z1.real();
+      a.c1i = z1.imag();
+      a.c2r = z.real();
+      a.c2i = z.imag();
+    }
+    rc = launch_apply(a, op->lattice, EPI_TAYLOR, s);
+    if (rc) return rc;
+  }
+  return OCTHIP_SUCCESS;
+}

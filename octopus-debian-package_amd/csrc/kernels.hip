@@ -1,0 +1,222 @@
+// Kernel instances and launchers: fused apply, projector bra (FP64 MFMA and plain).
+#include "kernels.h"
+
+namespace octhip {
+
+namespace {
+
+template <int N, int IDX, int EPI>
+int launch_apply_n(ApplyArgs& a, hipStream_t s) {
+  const int tiles = a.ntiles;
+  const int per = 8 * (a.chunk > 1 ? a.chunk : 1);
+  const int grid = ((tiles + per - 1) / per) * per;
+  ProfScope prof("k_apply", s);
+  if (a.tpp == 64)
+    hipLaunchKernelGGL((k_apply<N, IDX, true, EPI>), dim3(grid), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_apply<N, IDX, false, EPI>), dim3(grid), dim3(BLOCK), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_apply launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+template <int IDX, int EPI>
+int launch_apply_idx(ApplyArgs& a, hipStream_t s) {
+  switch (a.n) {
+    case 25: return launch_apply_n<25, IDX, EPI>(a, s);
+    case 19: return launch_apply_n<19, IDX, EPI>(a, s);
+    case 13: return launch_apply_n<13, IDX, EPI>(a, s);
+    case 7: return launch_apply_n<7, IDX, EPI>(a, s);
+    default: return launch_apply_n<0, IDX, EPI>(a, s);
+  }
+}
+
+}  // namespace
+
+int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s) {
+  if (a.np <= 0) return OCTHIP_SUCCESS;
+  int tpp = a.L < 64 ? a.L : 64;
+  int tshift = 0;
+  while ((1 << tshift) < tpp) ++tshift;
+  a.tpp = tpp;
+  a.tshift = tshift;
+  a.ppb = BLOCK / tpp;
+  a.ntiles = (a.np + a.ppb - 1) / a.ppb;
+  if (a.chunk < 1) a.chunk = 1;
+  if (lattice) {
+    return epi == EPI_STORE ? launch_apply_idx<IDX_LATTICE, EPI_STORE>(a, s)
+                            : launch_apply_idx<IDX_LATTICE, EPI_TAYLOR>(a, s);
+  }
+  return epi == EPI_STORE ? launch_apply_idx<IDX_RI, EPI_STORE>(a, s)
+                          : launch_apply_idx<IDX_RI, EPI_TAYLOR>(a, s);
+}
+
+// -------------------------------------------------------------- bra (plain) --
+// One workgroup per chunk of <= BRA_CHUNK sphere points of one matrix.  Threads are
+// (slice, lane): each slice sums every nsl-th sphere point of the chunk for one lane,
+// slices are then added in fixed order through LDS, so results are deterministic.
+constexpr int BRA_PJB = 8;  // projectors per register block
+
+__global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) {
+  __shared__ double red[BLOCK * BRA_PJB];
+  const int c = blockIdx.x;
+  const int4 ch = b.chunks[c];
+  const int4 mi = b.mat_info[ch.x];
+  const int npt = mi.x, npj = mi.y, moff = mi.z, mapoff = mi.w;
+  const int Lc = b.L < BLOCK ? b.L : BLOCK;
+  const int nsl = BLOCK / Lc;
+  const int sl = threadIdx.x / Lc;
+  const int lc = threadIdx.x - sl * Lc;
+  for (int lg = 0; lg < b.L; lg += Lc) {
+    const int l = lg + lc;
+    for (int jb = 0; jb < npj; jb += BRA_PJB) {
+      double acc[BRA_PJB];
+#pragma unroll
+      for (int jj = 0; jj < BRA_PJB; ++jj) acc[jj] = 0.0;
+      for (int q = ch.y + sl; q < ch.z; q += nsl) {
+        const int row = b.maps[mapoff + q] - 1;
+        const double x = b.psi[(size_t)row * b.L + l];
+#pragma unroll
+        for (int jj = 0; jj < BRA_PJB; ++jj)
+          if (jb + jj < npj) acc[jj] = fma(b.mats[moff + (size_t)(jb + jj) * npt + q], x, acc[jj]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < BRA_PJB; ++jj) red[(sl * BRA_PJB + jj) * Lc + lc] = acc[jj];
+      __syncthreads();
+      if (sl == 0) {
+        for (int jj = 0; jj < BRA_PJB && jb + jj < npj; ++jj) {
+          double t = 0.0;
+          for (int s2 = 0; s2 < nsl; ++s2) t += red[(s2 * BRA_PJB + jj) * Lc + lc];
+          b.partial[((size_t)c * b.pjmax + jb + jj) * b.L + l] = t;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// --------------------------------------------------------------- bra (MFMA) --
+// D(16 projectors x 16 lanes) += A(16 x 4 sphere points) * B(4 x 16 lanes) with
+// v_mfma_f64_16x16x4_f64.  Operand layout (wave64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator element r of lane l is D[4*r + l/16][l%16] (checked on the
+// device with an in-bounds probe).
+// A comes from the transposed projector table (16 lanes read consecutive doubles of one
+// sphere point's row), B from the state rows (16 lanes read 128 contiguous bytes).
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct BraMfmaArgs {
+  BraArgs b;
+  const double* __restrict__ pt;  // total_points x pjs
+  int pjs;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
+  __shared__ double red[4 * 16 * 16];
+  const BraArgs& b = m.b;
+  const int c = blockIdx.x;
+  const int4 ch = b.chunks[c];
+  const int4 mi = b.mat_info[ch.x];
+  const int npj = mi.y, mapoff = mi.w;
+  const int wave = threadIdx.x >> 6;
+  const int ln = threadIdx.x & 63;
+  const int kk = ln >> 4, ii = ln & 15;
+  const int nlt = b.L >> 4;                 // lane tiles of 16
+  // waves split lane tiles; if fewer tiles than waves, they also split the sphere points
+  const int nsl = nlt >= 4 ? 1 : 4 / nlt;   // slices over points
+  const int my_sl = nlt >= 4 ? 0 : wave / nlt;
+  const int t0 = nlt >= 4 ? wave : wave % nlt;
+  const int tstep = nlt >= 4 ? 4 : nlt;
+  for (int jb = 0; jb < npj; jb += 16) {
+    for (int lt = t0; lt < nlt; lt += tstep) {
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int col = lt * 16 + ii;
+      for (int q = ch.y + 4 * my_sl; q < ch.z; q += 4 * nsl) {
+        const int qk = q + kk;
+        double av = 0.0, bv = 0.0;
+        if (qk < ch.z) {
+          if (jb + ii < npj) av = m.pt[(size_t)(mapoff + qk) * m.pjs + jb + ii];
+          const int row = b.maps[mapoff + qk] - 1;
+          bv = b.psi[(size_t)row * b.L + col];
+        }
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+      if (nsl == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = jb + 4 * r + kk;
+          if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = acc[r];
+        }
+      } else {
+        // deterministic slice reduction through LDS
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[(wave * 16 + 4 * r + kk) * 16 + ii] = acc[r];
+        __syncthreads();
+        if (my_sl == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int jl = 4 * r + kk;
+            double t = 0.0;
+            for (int s2 = 0; s2 < nsl; ++s2) t += red[((s2 * nlt + t0) * 16 + jl) * 16 + ii];
+            const int j = jb + jl;
+            if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = t;
+          }
+        }
+      }
+    }
+  }
+}
+
+// proj[i, l] = scal[i] * sum_{chunks of matrix(i)} partial[chunk, i - pbase, l]
+__global__ void k_bra_reduce(const BraReduceArgs r) {
+  const size_t total = (size_t)r.nproj * r.L;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / r.L);
+    const int l = (int)(t - (size_t)i * r.L);
+    const int a = r.proj_mat[i];
+    const int j = i - r.mat_pbase[a];
+    double s = 0.0;
+    for (int c = r.mat_chunk0[a]; c < r.mat_chunk0[a + 1]; ++c)
+      s += r.partial[((size_t)c * r.pjmax + j) * r.L + l];
+    r.proj[t] = s * r.scals[i];
+  }
+}
+
+}  // namespace octhip
+
+namespace octhip {
+
+int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s) {
+  if (nchunk <= 0) return OCTHIP_SUCCESS;
+  ProfScope prof("k_bra", s);
+  hipLaunchKernelGGL(k_bra_plain, dim3(nchunk), dim3(BLOCK), 0, s, b);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra launch: %s", hipGetErrorString(e));
+  (void)mfma;
+  return OCTHIP_SUCCESS;
+}
+
+int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s) {
+  if (nchunk <= 0) return OCTHIP_SUCCESS;
+  BraMfmaArgs m{b, pt, pjs};
+  ProfScope prof("k_bra", s);
+  hipLaunchKernelGGL(k_bra_mfma, dim3(nchunk), dim3(BLOCK), 0, s, m);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra_mfma launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s) {
+  const size_t total = (size_t)r.nproj * r.L;
+  if (total == 0) return OCTHIP_SUCCESS;
+  int grid = (int)((total + BLOCK - 1) / BLOCK);
+  if (grid > 4096) grid = 4096;
+  ProfScope prof("k_bra_reduce", s);
+  hipLaunchKernelGGL(k_bra_reduce, dim3(grid), dim3(BLOCK), 0, s, r);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra_reduce launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+}  // namespace octhip

@@ -1,0 +1,133 @@
+// Internal declarations shared by the HIP translation units of liboctopus_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "octhip.h"
+
+namespace octhip {
+
+// largest packed row handled: 2^MAX_LDF doubles (pack_size_real(1) <= 1024)
+constexpr int MAX_LDF = 10;
+// largest stencil size of the device path
+constexpr int MAX_STENCIL = 64;
+// threads per workgroup of every streaming kernel
+constexpr int BLOCK = 256;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+bool initialised();
+int current_device();
+// device buffer of >= 2^MAX_LDF zero doubles (stands for zero-Dirichlet boundary rows)
+const double* zero_row();
+
+#define OCTHIP_HIP(call)                                                                   \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::octhip::fail(OCTHIP_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                           \
+  } while (0)
+
+#define OCTHIP_REQUIRE(cond, ...)                                  \
+  do {                                                             \
+    if (!(cond)) return ::octhip::fail(OCTHIP_ERR_INVALID, __VA_ARGS__); \
+  } while (0)
+
+#define OCTHIP_REQUIRE_INIT()                                                        \
+  do {                                                                               \
+    if (!::octhip::initialised())                                                    \
+      return ::octhip::fail(OCTHIP_ERR_STATE, "octhip_init has not been called");    \
+  } while (0)
+
+// RAII-free device allocation helpers: every handle frees what it owns in *_destroy.
+template <typename T>
+int upload(T** dst, const T* src, size_t count, hipStream_t s) {
+  *dst = nullptr;
+  if (count == 0) return OCTHIP_SUCCESS;
+  hipError_t e = hipMalloc((void**)dst, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(OCTHIP_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "upload: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+int dalloc(void** p, size_t bytes);
+void dfree(void* p);
+
+// Device timing of one kernel launch when profiling is on (events on the launch stream).
+class ProfScope {
+ public:
+  ProfScope(const char* region, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  int slot_ = -1;
+  hipStream_t s_;
+};
+
+}  // namespace octhip
+
+// ------------------------------------------------------------------ handles --
+struct octhip_nl_operator_s {
+  int n = 0;             // stencil size
+  int np = 0, np_part = 0;
+  int nri = 0;
+  int center = -1;       // stencil index of the zero offset, -1 if none
+  std::vector<double> w; // const weights
+  // lattice (site-map) indexing
+  bool lattice = false;
+  int dcube[octhip::MAX_STENCIL] = {0};  // site-index delta of stencil point j
+  int* d_cidx = nullptr;                 // np: site index of each inner point
+  int* d_cube = nullptr;                 // sites: point index or -1
+  size_t ncube = 0;
+  // run-compressed indexing (Octopus's own ri/rimap)
+  int* d_ri = nullptr;                   // nri*n
+  int* d_rimap = nullptr;                // np
+  // host copy of lattice coordinates (3*np_part), kept for schedules
+  std::vector<int> coords;
+  int xcd_chunk = 1;                     // blockIdx -> tile remap granularity
+};
+
+struct octhip_hamiltonian_s {
+  octhip_nl_operator_t kin = nullptr;
+  double mass = 1.0;
+  int np = 0;
+  double* d_v = nullptr;        // np local potential
+  double* d_xyz = nullptr;      // 3*np positions for the dipole term (optional)
+  double field[3] = {0, 0, 0};
+  bool has_field = false;
+  // projectors
+  int nmat = 0;
+  int max_nprojs = 0;
+  int full_projection_size = 0;
+  int total_points = 0;
+  int pjs = 0;                  // row stride of the transposed projector table
+  std::vector<int> offsets;     // 6*nmat host copy
+  double* d_matrices = nullptr; // column-major blocks
+  int* d_maps = nullptr;        // 1-based
+  double* d_scals = nullptr;
+  double* d_pt = nullptr;       // total_points x pjs (row-major, transposed blocks)
+  int* d_pos = nullptr;         // np+1 CSR of sphere points per mesh point
+  int2* d_ent = nullptr;        // (global sphere point, matrix)
+  int* d_mat_pbase = nullptr;   // nmat: offset of the matrix's projections
+  int* d_mat_npj = nullptr;     // nmat: nprojs
+  // bra chunking: chunk c = (matrix, first sphere point, last+1)
+  int nchunk = 0;
+  int4* d_chunks = nullptr;     // {matrix, q0, q1, 0}
+  int* d_mat_chunk0 = nullptr;  // nmat+1 first chunk of each matrix
+  int4* d_mat_info = nullptr;   // nmat: {npoints, nprojs, matrix offset, map offset}
+  int* d_proj_mat = nullptr;    // projection index -> matrix
+  // work buffers sized for the largest ldf seen
+  int work_ldf = -1;
+  double* d_partial = nullptr;  // nchunk * max_nprojs * L
+  double* d_proj = nullptr;     // full_projection_size * L
+  double* d_y[2] = {nullptr, nullptr};  // Taylor intermediates, np rows each
+  int y_ldf = -1;
+  int use_mfma = 1;
+};

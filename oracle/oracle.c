@@ -1,0 +1,199 @@
+/*
+ * oracle.c -- CPU restatement of Octopus's H-apply path, used ONLY as the checker of the
+ * device path (tests, smoke check) and as the timed CPU baseline.  It is test
+ * infrastructure: the product library never links or calls it.
+ *
+ * Every function names the reference lines it restates.  Arithmetic conventions, fixed so
+ * the device kernels can be compared bit for bit where the order of operations is the
+ * same:
+ *   - the stencil sum runs over j = 0..n-1 with fused multiply-adds from 0, one lane at
+ *     a time, exactly like the AVX leaf (src/grid/operate_inc.c:76-118, VEC_FMA);
+ *   - hpsi += v*psi is one fused multiply-add;
+ *   - the projector contractions sum over sphere points (bra) or projectors (ket) in
+ *     increasing order with fused multiply-adds (the reference calls BLAS dgemm there,
+ *     whose order is unspecified -- see the header of the tests for how it is pinned);
+ *   - complex axpy on interleaved lanes: re += zr*xr - zi*xi, im += zr*xi + zi*xr,
+ *     products rounded separately.
+ * Built with -ffp-contract=off so the compiler adds no contractions of its own.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(p, k, L) ((size_t)(p) * (size_t)(L) + (size_t)(k))
+
+/* src/grid/operate_inc.c:29-342 (doperate_ri_vec): runs l, points i in
+ * [imin[l], imax[l]), lanes k < 2^ldf: fo[i,k] = sum_j w[j] * fi[i + ri[l*n+j], k]. */
+void oracle_operate_ri_vec(int n, const double* w, int nri, const int* ri, const int* imin,
+                           const int* imax, const double* fi, int ldf, double* fo) {
+  const int L = 1 << ldf;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int l = 0; l < nri; ++l) {
+    const int* index = ri + (size_t)l * n;
+    for (int i = imin[l]; i < imax[l]; ++i) {
+      for (int k = 0; k < L; ++k) {
+        double a = 0.0;
+        for (int j = 0; j < n; ++j) a = fma(w[j], fi[IDX(index[j] + i, k, L)], a);
+        fo[IDX(i, k, L)] = a;
+      }
+    }
+  }
+}
+
+/* boundaries_set, zero Dirichlet branch (src/grid/boundaries_inc.F90:275-363):
+ * rows np..np_part-1 of a packed batch are set to zero. */
+void oracle_zero_boundary(int np, int np_part, int ldf, double* f) {
+  const size_t L = (size_t)1 << ldf;
+  memset(f + (size_t)np * L, 0, sizeof(double) * (size_t)(np_part - np) * L);
+}
+
+/* X(hamiltonian_elec_base_local_sub), packed real-potential branch
+ * (src/hamiltonian/hamiltonian_elec_base_inc.F90:231-241): hpsi(k,p) += v(p) psi(k,p).
+ * field != NULL adds the dipole term sum_d field[d]*x_d(p) to v (laser_potential,
+ * src/hamiltonian/lasers.F90:823-853, + sign). */
+void oracle_vlocal(int np, int ldf, const double* v, const double* xyz, const double* field,
+                   const double* psi, double* hpsi) {
+  const int L = 1 << ldf;
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < np; ++p) {
+    double vv = v[p];
+    if (field)
+      vv = vv + (field[0] * xyz[3 * (size_t)p] + field[1] * xyz[3 * (size_t)p + 1] +
+                 field[2] * xyz[3 * (size_t)p + 2]);
+    for (int k = 0; k < L; ++k) hpsi[IDX(p, k, L)] = fma(vv, psi[IDX(p, k, L)], hpsi[IDX(p, k, L)]);
+  }
+}
+
+/* X(hamiltonian_elec_base_nlocal_start), CPU packed branch
+ * (src/hamiltonian/hamiltonian_elec_base_inc.F90:618-786): for each matrix, gather
+ * lpsi(k, q) = psi(k, map(q)), proj(k, j) = sum_q lpsi(k,q) P(q,j) (dgemm 'N','N'),
+ * then proj(k, j) *= scal(j).  offsets use build_accel's layout
+ * (src/hamiltonian/hamiltonian_elec_base.F90:785-816); maps are 1-based. */
+void oracle_nlocal_start(int nmat, const int* offsets, const double* matrices, const int* maps,
+                         const double* scals, int ldf, const double* psi, double* proj) {
+  const int L = 1 << ldf;
+  for (int a = 0; a < nmat; ++a) {
+    const int* o = offsets + 6 * (size_t)a;
+    const int npt = o[0], npj = o[1], moff = o[2], mapoff = o[3], soff = o[4];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < npj; ++j)
+      for (int k = 0; k < L; ++k) {
+        double s = 0.0;
+        for (int q = 0; q < npt; ++q)
+          s = fma(psi[IDX(maps[mapoff + q] - 1, k, L)], matrices[moff + (size_t)j * npt + q], s);
+        proj[IDX(soff + j, k, L)] = s * scals[soff + j];
+      }
+  }
+}
+
+/* X(hamiltonian_elec_base_nlocal_finish) (src/hamiltonian/hamiltonian_elec_base_inc.F90:
+ * 859-943): per matrix in order, psi_loc(k, q) = sum_j proj(k, j) P(q, j) (dgemm 'N','T'),
+ * then hpsi(k, map(q)) += psi_loc(k, q).  Matrices are applied one after the other, so
+ * points shared by several spheres accumulate in matrix order. */
+void oracle_nlocal_finish(int nmat, const int* offsets, const double* matrices, const int* maps,
+                          int ldf, const double* proj, double* hpsi) {
+  const int L = 1 << ldf;
+  for (int a = 0; a < nmat; ++a) {
+    const int* o = offsets + 6 * (size_t)a;
+    const int npt = o[0], npj = o[1], moff = o[2], mapoff = o[3], soff = o[4];
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < npt; ++q) {
+      const int p = maps[mapoff + q] - 1;
+      for (int k = 0; k < L; ++k) {
+        double t = 0.0;
+        for (int j = 0; j < npj; ++j) t = fma(matrices[moff + (size_t)j * npt + q], proj[IDX(soff + j, k, L)], t);
+        hpsi[IDX(p, k, L)] = hpsi[IDX(p, k, L)] + t;
+      }
+    }
+  }
+}
+
+static int full_projection_size(int nmat, const int* offsets) {
+  int s = 0;
+  for (int a = 0; a < nmat; ++a)
+    if (offsets[6 * a + 4] + offsets[6 * a + 1] > s) s = offsets[6 * a + 4] + offsets[6 * a + 1];
+  return s;
+}
+
+/* X(hamiltonian_elec_apply_batch) for the terms kinetic + local + KB nonlocal
+ * (src/hamiltonian/hamiltonian_elec_inc.F90:77-261): set_bc -> boundaries_set; projections
+ * from psi; hpsi = (-1/2m) L psi (weights pre-multiplied, nl_operator_inc.F90:72-80);
+ * hpsi += v psi; hpsi += nonlocal. */
+int oracle_hamiltonian_apply(const oracle_ham_t* h, double* psi, double* hpsi, int ldf, int terms,
+                             int set_bc) {
+  const int L = 1 << ldf;
+  if (set_bc) oracle_zero_boundary(h->np, h->np_part, ldf, psi);
+  double* proj = NULL;
+  const int nl = (terms & ORACLE_TERM_NON_LOCAL) && h->nmat > 0;
+  if (nl) {
+    const int fps = full_projection_size(h->nmat, h->offsets);
+    proj = (double*)malloc(sizeof(double) * (size_t)fps * L);
+    if (!proj) return -1;
+    oracle_nlocal_start(h->nmat, h->offsets, h->matrices, h->maps, h->scals, ldf, psi, proj);
+  }
+  if (terms & ORACLE_TERM_KINETIC) {
+    double wre[128];
+    if (h->n > 128) return -1;
+    for (int j = 0; j < h->n; ++j) wre[j] = h->w[j] * (-0.5 / h->mass);
+    oracle_operate_ri_vec(h->n, wre, h->nri, h->ri, h->rimap_inv, h->rimap_inv + 1, psi, ldf, hpsi);
+  } else {
+    memset(hpsi, 0, sizeof(double) * (size_t)h->np * L);
+  }
+  if ((terms & ORACLE_TERM_LOCAL) && h->v) oracle_vlocal(h->np, ldf, h->v, h->xyz, h->field, psi, hpsi);
+  if (nl) {
+    oracle_nlocal_finish(h->nmat, h->offsets, h->matrices, h->maps, ldf, proj, hpsi);
+    free(proj);
+  }
+  return 0;
+}
+
+/* batch_axpy on the inner rows with a complex factor (src/grid/batch_ops_inc.F90:23-106),
+ * complex lanes interleaved. */
+void oracle_zaxpy(int np, int ldf, double zr, double zi, const double* x, double* y) {
+  const size_t L = (size_t)1 << ldf;
+  const size_t count = (size_t)np * L;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < count; i += 2) {
+    const double xr = x[i], xi = x[i + 1];
+    y[i] = y[i] + (zr * xr - zi * xi);
+    y[i + 1] = y[i + 1] + (zr * xi + zi * xr);
+  }
+}
+
+/* exponential_taylor_series_batch (src/electrons/exponential.F90:677-757):
+ * zfact = 1; for k = 1..order: zfact *= (-i*deltat)/k; hpsi1 = H (k == 1 ? psi : psi1);
+ * psi += zfact * hpsi1; psi1 = hpsi1.  imag_time: deltat -> i*deltat
+ * (exponential.F90:609-619).  Work arrays are np_part rows like the reference's clones;
+ * every H-apply sets the zero boundary of its input (set_bc default). */
+int oracle_taylor(const oracle_ham_t* h, double* psi, int ldf, double deltat, int order, int imag_time) {
+  const size_t L = (size_t)1 << ldf;
+  const size_t rows = (size_t)h->np_part;
+  double* psi1 = (double*)calloc(rows * L, sizeof(double));
+  double* hpsi1 = (double*)calloc(rows * L, sizeof(double));
+  if (!psi1 || !hpsi1) {
+    free(psi1);
+    free(hpsi1);
+    return -1;
+  }
+  const double dtr = imag_time ? 0.0 : deltat, dti = imag_time ? deltat : 0.0;
+  /* -i * deltat with complex arithmetic, then zfact = (zfact * that) / k */
+  const double mr = -0.0, mi = -1.0;
+  const double sr = mr * dtr - mi * dti, si = mr * dti + mi * dtr;
+  double zr = 1.0, zi = 0.0;
+  int rc = 0;
+  for (int k = 1; k <= order && rc == 0; ++k) {
+    const double tr = zr * sr - zi * si, ti = zr * si + zi * sr;
+    zr = tr / k;
+    zi = ti / k;
+    rc = oracle_hamiltonian_apply(h, k == 1 ? psi : psi1, hpsi1, ldf, ORACLE_TERM_ALL, 1);
+    oracle_zaxpy(h->np, ldf, zr, zi, hpsi1, psi);
+    if (k != order) memcpy(psi1, hpsi1, sizeof(double) * (size_t)h->np * L);
+  }
+  free(psi1);
+  free(hpsi1);
+  return rc;
+}
+
+int oracle_version(void) { return 1; }

@@ -1,0 +1,177 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars (the tolerance of each check is written in it):
+* stencil and local potential: bit-identical -- both sides sum the 25 terms in stencil
+  order with fused multiply-adds;
+* anything with the KB projectors: max relative error <= 1e-12 per state (max|x-y|/max|y|),
+  since the device bra sums sphere points in a different (blocked / MFMA) order;
+* Taylor runs: per-state error and norm drift <= 1e-10 against the oracle's propagator.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import max_rel_err, random_packed, small_sphere
+from octhip import api, workloads
+from octhip.stencil import star_points
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(gpu, a):
+    return gpu.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def run_operate(gpu, op_dev, fin, factor):
+    L = fin.shape[1]
+    fi = api.Batch(L, fin.shape[0], False, ff=to_dev(gpu, fin))
+    fo = api.Batch(L, op_dev.np, False)
+    api.nl_operator_operate_batch(op_dev, fi, fo, factor)
+    gpu.cuda.synchronize()
+    return fo.ff.cpu().numpy()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 8, 64, 128])
+@pytest.mark.parametrize("lattice", [True, False])
+def test_stencil_bit_exact_sphere(gpu, lanes, lattice):
+    print(f"[stencil sphere lanes={lanes} lattice={lattice}]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.25)
+    op = api.NLOperatorDevice(s.op, star_points(4) if lattice else None)
+    assert op.uses_lattice == lattice
+    f = random_packed(s.np_part, lanes, seed=lanes)
+    got = run_operate(gpu, op, f, -0.5)
+    ref = O.operate_ri_vec(s.op, f, -0.5)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("chunk", [1, 4, 64])
+def test_stencil_bit_exact_cube_workload(gpu, chunk):
+    """The 64^3 cube workload (8 Gaussian sums, pack 8) at full size; schedules only
+    reorder tiles, so every schedule is bit-identical."""
+    print(f"[stencil cube64 chunk={chunk}]", flush=True)
+    s = workloads.laplacian_system()
+    f = workloads.gaussian_functions(s, 8)
+    packed = workloads.pack(f, s.np_part, 8)
+    op = api.NLOperatorDevice(s.op, star_points(4))
+    op.set_schedule(chunk)
+    got = run_operate(gpu, op, packed, 1.0)
+    ref = O.operate_ri_vec(s.op, packed, 1.0)
+    assert np.array_equal(got, ref)
+
+
+def run_h(gpu, s, z, terms=api.TERM_ALL, set_bc=True, lattice=True, bra=1, pack_size=None):
+    hm = api.HamiltonianDevice.from_system(s, lattice=lattice)
+    hm.set_bra_mode(bra)
+    psib = api.Batch.from_states(z, s.np_part, pack_size)
+    hpsib = api.Batch(psib.nst, s.np, psib.is_complex, psib.pack_size)
+    api.hamiltonian_elec_apply_batch(hm, psib, hpsib, terms, set_bc)
+    gpu.cuda.synchronize()
+    return hpsib.ff.cpu().numpy()
+
+
+def oracle_h(s, z, terms=O.TERM_ALL, set_bc=True, pack_size=None):
+    psi = workloads.pack(z, s.np_part, pack_size)
+    return O.Hamiltonian(s.op, s.v, s.proj).apply(psi, terms, set_bc)
+
+
+def states(s, nst, seed=0):
+    return workloads.gaussian_envelope_states(s, nst, width=2.0, first=seed)
+
+
+@pytest.mark.parametrize("terms", [api.TERM_KINETIC, api.TERM_LOCAL_POTENTIAL,
+                                   api.TERM_KINETIC | api.TERM_LOCAL_POTENTIAL])
+def test_hamiltonian_local_terms_bit_exact(gpu, terms):
+    print(f"[hamiltonian terms={terms}]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.25, natoms=4)
+    z = states(s, 16)
+    assert np.array_equal(run_h(gpu, s, z, terms), oracle_h(s, z, terms))
+
+
+@pytest.mark.parametrize("bra", [0, 1])
+@pytest.mark.parametrize("nst", [1, 3, 16, 32, 64])
+def test_hamiltonian_full(gpu, bra, nst):
+    print(f"[hamiltonian full bra={bra} nst={nst}]", flush=True)
+    s = small_sphere(radius=3.5, spacing=0.25, natoms=5, rc=1.5)
+    z = states(s, nst)
+    got, ref = run_h(gpu, s, z, bra=bra), oracle_h(s, z)
+    assert max_rel_err(got, ref) <= 1e-12
+
+
+def test_hamiltonian_real_batch(gpu):
+    print("[hamiltonian real]", flush=True)
+    s = small_sphere(radius=3.5, spacing=0.25, natoms=5, rc=1.5)
+    z = states(s, 8).real.copy()
+    assert max_rel_err(run_h(gpu, s, z), oracle_h(s, z)) <= 1e-12
+
+
+def test_hamiltonian_boundary_rows_as_given(gpu):
+    """set_bc = .false.: boundary rows are read as the caller left them."""
+    print("[hamiltonian set_bc=0]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.25, natoms=3)
+    z = states(s, 8)
+    psi = workloads.pack(z, s.np_part)
+    psi[s.np:] = np.random.default_rng(1).standard_normal(psi[s.np:].shape)
+    hm = api.HamiltonianDevice.from_system(s)
+    psib = api.Batch(8, s.np_part, True, ff=to_dev(gpu, psi))
+    hpsib = api.Batch(8, s.np, True)
+    api.hamiltonian_elec_apply_batch(hm, psib, hpsib, api.TERM_ALL, set_bc=False)
+    gpu.cuda.synchronize()
+    ref = O.Hamiltonian(s.op, s.v, s.proj).apply(psi.copy(), O.TERM_ALL, set_bc=False)
+    assert max_rel_err(hpsib.ff.cpu().numpy(), ref) <= 1e-12
+    # set_bc = .true. zeroes psi's boundary rows, like boundaries_set
+    api.hamiltonian_elec_apply_batch(hm, psib, hpsib, api.TERM_ALL, set_bc=True)
+    gpu.cuda.synchronize()
+    assert not np.any(psib.ff.cpu().numpy()[s.np:])
+
+
+def test_hamiltonian_no_projectors(gpu):
+    print("[hamiltonian nmat=0]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.25, natoms=0)
+    z = states(s, 4)
+    assert np.array_equal(run_h(gpu, s, z), oracle_h(s, z))
+
+
+def test_hamiltonian_workload_batch(gpu):
+    """One 32-state batch of the R=12, h=0.25 sphere workload (462,781 points)."""
+    print("[hamiltonian sphere R=12 batch]", flush=True)
+    s = workloads.hamiltonian_system()
+    z = workloads.gaussian_envelope_states(s, 32)
+    got, ref = run_h(gpu, s, z), oracle_h(s, z)
+    assert max_rel_err(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("imag_time", [False, True])
+def test_taylor_step(gpu, order, imag_time):
+    print(f"[taylor order={order} imag={imag_time}]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.3, natoms=3, rc=1.2)
+    z = workloads.orthonormalise(states(s, 8), s.mesh.vol_pp)
+    psi = workloads.pack(z, s.np_part)
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, 0.01, order, imag_time)
+    hm = api.HamiltonianDevice.from_system(s)
+    psib = api.Batch.from_states(z, s.np_part)
+    api.Exponential(order).apply_batch(hm, psib, 0.01, imag_time)
+    gpu.cuda.synchronize()
+    assert max_rel_err(psib.ff.cpu().numpy()[: s.np], psi[: s.np]) <= 1e-12
+
+
+def test_taylor_run_drift(gpu):
+    """50 Taylor-4 steps at a stable dt: state error and norm drift <= 1e-10."""
+    print("[taylor 50 steps]", flush=True)
+    s = small_sphere(radius=4.0, spacing=0.3, natoms=4, rc=1.5)
+    z = workloads.orthonormalise(states(s, 16), s.mesh.vol_pp)
+    psi = workloads.pack(z, s.np_part)
+    oh = O.Hamiltonian(s.op, s.v, s.proj)
+    hm = api.HamiltonianDevice.from_system(s)
+    psib = api.Batch.from_states(z, s.np_part)
+    te = api.Exponential(4)
+    for _ in range(50):
+        oh.taylor(psi, 0.02)
+        te.apply_batch(hm, psib, 0.02)
+    gpu.cuda.synchronize()
+    got = workloads.unpack(psib.ff.cpu().numpy(), 16, s.np, True)
+    ref = workloads.unpack(psi, 16, s.np, True)
+    err = np.max(np.abs(got - ref), axis=1) / np.max(np.abs(ref), axis=1)
+    assert np.max(err) <= 1e-10
+    ng, nr = np.sum(np.abs(got) ** 2, axis=1), np.sum(np.abs(ref) ** 2, axis=1)
+    assert np.max(np.abs(ng - nr) / nr) <= 1e-10

@@ -1,0 +1,263 @@
+"""Pinning the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+The reference cannot be built here (its C leaf includes the configure-generated
+config.h; the Fortran needs BLAS/LAPACK/GSL/libxc/FFTW), so every oracle function is
+pinned by what the mathematics fixes, chosen so a plausible mistake fails one check:
+
+* stencil (oracle_operate_ri_vec, src/grid/operate_inc.c:29-342):
+  - exactness on per-axis polynomials of degree <= 9 (the order-4 star is exact for them);
+  - a brute-force 3-D stencil through the lattice lookup, independent of ri/rimap_inv;
+  - symmetry <u, L v> = <L u, v> with zero boundary, zero row sum in the interior;
+  - the reference's own Laplacian test (testsuite/components/03-derivatives_3d.test),
+    within the precision the reference itself applies;
+* local potential: exact pointwise identity;
+* KB projectors (nlocal_start/finish): a dense sum_a P_a diag(scal_a) P_a^T assembled
+  independently, and its symmetry;
+* full H-apply: a dense H assembled column by column equals K + diag(v) + V_nl built
+  independently, and is symmetric;
+* Taylor (src/electrons/exponential.F90:677-757): pointwise closed form for a diagonal H,
+  dense sum_{n<=4} (-i dt H)^n/n! psi, and the non-unitarity |p(i theta)|^2 =
+  1 - theta^6/72 + theta^8/576 on eigenvectors;
+* planted mistakes (in the oracle's inputs, on the CPU) that these checks must catch.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import brute_laplacian, small_sphere
+from octhip import mesh as M, stencil as S, workloads
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def tiny_system(natoms=3):
+    return small_sphere(radius=1.5, spacing=0.3, natoms=natoms, nwells=3, rc=0.9)
+
+
+def dense_parts(system):
+    """K (-1/2 L, zero BC), diag(v), V_nl assembled without the oracle."""
+    m, op = system.mesh, system.op
+    npt = m.np
+    nb = S.neighbours(m, S.star_points(4))
+    K = np.zeros((npt, npt))
+    for j in range(op.n):
+        inner = nb[:, j] < npt
+        rows = np.nonzero(inner)[0]
+        np.add.at(K, (rows, nb[inner, j]), -0.5 * op.w[j])
+    V = np.diag(system.v)
+    Vnl = np.zeros((npt, npt))
+    if system.proj is not None:
+        for a in range(system.proj.nmat):
+            P, mp, sc = system.proj.matrix(a)
+            Pe = np.zeros((npt, P.shape[1]))
+            Pe[mp - 1] = P
+            Vnl += Pe @ np.diag(sc) @ Pe.T
+    return K, V, Vnl
+
+
+def dense_from_oracle(system, terms=O.TERM_ALL):
+    m = system.mesh
+    npt = m.np
+    L = 1 << int(np.ceil(np.log2(npt)))
+    psi = np.zeros((m.np_part, L))
+    psi[np.arange(npt), np.arange(npt)] = 1.0
+    h = O.Hamiltonian(system.op, system.v, system.proj)
+    return h.apply(psi, terms)[:, :npt]
+
+
+# ----------------------------------------------------------------- stencil --
+def test_weights_are_textbook_coefficients():
+    c = S.central_d2_coefficients(4)
+    assert [str(x) for x in c] == ["-205/72", "8/5", "-1/5", "8/315", "-1/560"]
+    w = S.weights_closed_form(4, 0.3)
+    assert np.max(np.abs(w - S.weights_polyfit(4, 0.3)) / np.abs(w)) < 1e-12
+    assert abs(np.sum(w)) < 1e-10 * np.max(np.abs(w))
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 5, 8, 9])
+def test_stencil_exact_on_polynomials(deg):
+    m = M.cube_mesh(20, 0.25, S.star_points(4), states_block_size=8)
+    op = S.nl_operator_build(m, 4)
+    x = m.x
+    cs = [1.0, -0.7, 0.4]
+    f = sum(c * (x[:, d] / 2.0) ** deg for d, c in enumerate(cs))
+    ex = sum(c * deg * (deg - 1) * (x[:, d] / 2.0) ** max(deg - 2, 0) / 4.0 for d, c in enumerate(cs))
+    out = O.operate_ri_vec(op, f[:, None].copy())[:, 0]
+    nb = S.neighbours(m, S.star_points(4))
+    interior = np.all(nb < m.np, axis=1)
+    err = np.max(np.abs(out[interior] - ex[: m.np][interior]))
+    assert err <= 1e-9 * max(1.0, np.max(np.abs(f)))
+
+
+def test_stencil_matches_brute_force_and_run_compression():
+    s = small_sphere(radius=2.5, spacing=0.3)
+    r = np.random.default_rng(7)
+    f = r.standard_normal((s.np_part, 8))
+    out = O.operate_ri_vec(s.op, f, -0.5)
+    ref = brute_laplacian(s.mesh, s.op.w, f, -0.5)
+    assert np.max(np.abs(out - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # every run's offsets reproduce the lattice neighbours
+    nb = S.neighbours(s.mesh, S.star_points(4))
+    assert np.array_equal(np.arange(s.np)[:, None] + s.op.ri[s.op.rimap], nb)
+
+
+def test_stencil_symmetric_and_zero_row_sum():
+    s = small_sphere(radius=2.5, spacing=0.3)
+    r = np.random.default_rng(8)
+    u = r.standard_normal((s.np_part, 1))
+    v = r.standard_normal((s.np_part, 1))
+    u[s.np:] = 0
+    v[s.np:] = 0
+    Lu = O.operate_ri_vec(s.op, u)[:, 0]
+    Lv = O.operate_ri_vec(s.op, v)[:, 0]
+    a, b = np.dot(u[: s.np, 0], Lv), np.dot(Lu, v[: s.np, 0])
+    assert abs(a - b) <= 1e-12 * (abs(a) + abs(b))
+    one = np.ones((s.np_part, 1))
+    L1 = O.operate_ri_vec(s.op, one)[:, 0]
+    assert np.max(np.abs(L1)) < 1e-9
+
+
+def _derivatives_golden():
+    return json.loads((GOLDEN / "derivatives_3d.json").read_text())
+
+
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_reference_derivatives_3d_laplacian(kind):
+    """testsuite/components/03-derivatives_3d.test (input 03-derivatives_3d.01.inp) and
+    the test routine src/grid/derivatives_inc.F90:378-501: f = b exp(-a r^2) + c on all
+    np_part points, res = 2 * (1/2 L f) - exact, error = ||res||_2 sqrt(h^3)."""
+    g = _derivatives_golden()
+    R, h = g["radius"], g["spacing"]
+    m = M.sphere_mesh(R, h, S.star_points(4), states_block_size=g["states_block_size"])
+    op = S.nl_operator_build(m, 4, S.weights_polyfit(4, h))
+    x = m.x
+    r2 = np.einsum("ij,ij->i", x, x)
+    aa, bb, cc = 1.0 / R, 10.0, 100.0
+    if kind == "complex":
+        aa, bb, cc = aa + 0.01j, bb * np.exp(0.345j), cc - 50j
+    ff = bb * np.exp(-aa * r2) + cc
+    fi = np.stack([ff.real, ff.imag], 1).copy() if kind == "complex" else ff.real[:, None].copy()
+    res = O.operate_ri_vec(op, fi, 0.5)
+    res = res[:, 0] + 1j * res[:, 1] if kind == "complex" else res[:, 0]
+    r2i = r2[: m.np]
+    exact = 4 * aa ** 2 * bb * r2i * np.exp(-aa * r2i) - 3 * 2 * aa * bb * np.exp(-aa * r2i)
+    err = np.linalg.norm(2 * res - exact) * math.sqrt(h ** 3)
+    ref = g["laplacian_error"][kind]
+    assert abs(err - ref) <= g["precision"]          # the reference's own criterion
+    assert err < ref                                 # exact-order star is at least as accurate
+
+
+# ------------------------------------------------------------ local / KB ----
+def test_local_potential_is_pointwise_product():
+    s = tiny_system()
+    r = np.random.default_rng(3)
+    psi = r.standard_normal((s.np_part, 4))
+    out = O.Hamiltonian(s.op, s.v, None).apply(psi.copy(), O.TERM_LOCAL)
+    assert np.array_equal(out, s.v[:, None] * psi[: s.np])
+
+
+def test_nonlocal_equals_dense_projector_sum():
+    s = tiny_system(natoms=4)
+    _, _, Vnl = dense_parts(s)
+    assert np.max(np.abs(Vnl - Vnl.T)) <= 1e-15 * np.max(np.abs(Vnl))
+    H = dense_from_oracle(s, O.TERM_NON_LOCAL)
+    assert np.max(np.abs(H - Vnl)) <= 1e-13 * np.max(np.abs(Vnl))
+
+
+def test_full_h_equals_dense_assembly_and_is_symmetric():
+    s = tiny_system(natoms=4)
+    K, V, Vnl = dense_parts(s)
+    H = dense_from_oracle(s)
+    Href = K + V + Vnl
+    assert np.max(np.abs(H - Href)) <= 1e-13 * np.max(np.abs(Href))
+    assert np.max(np.abs(H - H.T)) <= 1e-13 * np.max(np.abs(H))
+
+
+# ------------------------------------------------------------------ Taylor --
+def taylor_dense(H, psi, dt, order=4):
+    out = psi.copy()
+    term = psi.copy()
+    for k in range(1, order + 1):
+        term = (-1j * dt / k) * (H @ term)
+        out = out + term
+    return out
+
+
+def test_taylor_diagonal_closed_form():
+    s = tiny_system(natoms=0)
+    op0 = S.NLOperator(**{**s.op.__dict__, "w": np.zeros_like(s.op.w)})
+    h = O.Hamiltonian(op0, s.v, None)
+    r = np.random.default_rng(5)
+    z = (r.standard_normal((s.np, 2)) + 1j * r.standard_normal((s.np, 2)))
+    psi = workloads.pack(z.T, s.np_part, 2)
+    dt = 0.37
+    h.taylor(psi, dt, 4)
+    got = workloads.unpack(psi, 2, s.np, True).T
+    a = -1j * dt * s.v
+    p = 1 + a + a ** 2 / 2 + a ** 3 / 6 + a ** 4 / 24
+    assert np.max(np.abs(got - p[:, None] * z)) <= 1e-14 * np.max(np.abs(z))
+
+
+def test_taylor_matches_dense_series():
+    s = tiny_system(natoms=3)
+    K, V, Vnl = dense_parts(s)
+    H = K + V + Vnl
+    r = np.random.default_rng(6)
+    z = r.standard_normal(s.np) + 1j * r.standard_normal(s.np)
+    psi = workloads.pack(z[None, :], s.np_part, 1)
+    dt = 0.02
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, dt, 4)
+    got = workloads.unpack(psi, 1, s.np, True)[0]
+    ref = taylor_dense(H, z, dt)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_taylor_non_unitarity_on_eigenvectors():
+    s = tiny_system(natoms=3)
+    K, V, Vnl = dense_parts(s)
+    E, U = np.linalg.eigh(K + V + Vnl)
+    dt = 0.1
+    pick = [0, len(E) // 2, len(E) - 1]
+    z = U[:, pick].T.astype(np.complex128)
+    psi = workloads.pack(z, s.np_part, 4)
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, dt, 4)
+    got = workloads.unpack(psi, 3, s.np, True)
+    th = dt * E[pick]
+    expect = 1 - th ** 6 / 72 + th ** 8 / 576
+    assert np.max(np.abs(np.sum(np.abs(got) ** 2, axis=1) - expect)) <= 1e-11 * np.max(expect)
+
+
+# -------------------------------------------------------- planted mistakes --
+def test_planted_mistakes_are_caught():
+    s = tiny_system(natoms=3)
+    K, V, Vnl = dense_parts(s)
+    Href = K + V + Vnl
+    # (1) one stencil weight with the wrong sign
+    w_bad = s.op.w.copy()
+    w_bad[1] = -w_bad[1]
+    bad = S.NLOperator(**{**s.op.__dict__, "w": w_bad})
+    s_bad = workloads.System(mesh=s.mesh, op=bad, v=s.v, proj=s.proj, seed=0, name="bad")
+    assert np.max(np.abs(dense_from_oracle(s_bad) - Href)) > 1e-3
+    # (2) projector columns swapped between two projectors of one atom (transposed operand)
+    pr = s.proj
+    mats = pr.matrices.copy()
+    P, _, _ = pr.matrix(0)
+    o = pr.offsets[0]
+    blk = P[:, [1, 0, 2, 3]]
+    mats[o[2]: o[2] + P.size] = blk.T.ravel()
+    import dataclasses
+    pr_bad = dataclasses.replace(pr, matrices=mats, scals=pr.scals * np.r_[1.3, np.ones(len(pr.scals) - 1)])
+    s_bad = workloads.System(mesh=s.mesh, op=s.op, v=s.v, proj=pr_bad, seed=0, name="bad")
+    assert np.max(np.abs(dense_from_oracle(s_bad) - Href)) > 1e-8
+    # (3) a dropped Taylor term (order 3 instead of 4) fails the dense series check
+    r = np.random.default_rng(9)
+    z = r.standard_normal(s.np) + 1j * r.standard_normal(s.np)
+    psi = workloads.pack(z[None, :], s.np_part, 1)
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, 0.5, 3)
+    got = workloads.unpack(psi, 1, s.np, True)[0]
+    ref = taylor_dense(Href, z, 0.5)
+    assert np.max(np.abs(got - ref)) > 1e-6 * np.max(np.abs(ref))
