@@ -33,6 +33,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def tuning(args) -> dict:
+    """Tuning knobs given on the command line (unset ones keep the library defaults)."""
+    t = {}
+    for k in ("kernel", "xcd_chunk", "gather_tile", "zseg", "lane_chunk", "lanes_per_thread"):
+        v = getattr(args, k)
+        if v is not None:
+            t[k] = v
+    return t
+
+
 def hamiltonian_bytes(s, nst: int, complex_: bool = True) -> int:
     """Compulsory HBM bytes of one H-apply of a batch: psi read over np_part rows, H psi
     written over np rows, v, the projector matrices, maps and their points, weights."""
@@ -71,10 +81,8 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     log("building the R=12 h=0.25 sphere workload")
     s = workloads.hamiltonian_system()
     nst_total, pack = 64, 32
-    hm = api.HamiltonianDevice.from_system(s)
-    if args.xcd_chunk:
-        hm.kinetic.set_schedule(args.xcd_chunk)
-    hm.set_bra_mode(args.bra)
+    hm = api.HamiltonianDevice.from_system(s, lattice=not args.no_lattice)
+    hm.tune(bra_mode=args.bra, **tuning(args))
     batches, outs = [], []
     for b in range(nst_total // pack):
         z = workloads.gaussian_envelope_states(s, pack, first=b * pack)
@@ -99,7 +107,7 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     _lib.profiling_stop()
     ms_step = (t1 - t0) * 1e3 / args.steps
     kern = {}
-    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_zero_rows"):
+    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_zero_rows"):
         tot, n = _lib.profiling_get(k)
         kern[k] = {"avg_us": tot * 1e3 / n if n else None, "launches": n}
     bytes_batch = hamiltonian_bytes(s, pack)
@@ -137,8 +145,7 @@ def bench_laplacian(args, torch, api, _lib, workloads):
     fi = api.Batch.from_states(f, s.np_part, 8)
     fo = api.Batch(8, s.np, False, 8)
     op = api.NLOperatorDevice(s.op, star_points(4))
-    if args.xcd_chunk:
-        op.set_schedule(args.xcd_chunk)
+    op.tune(**tuning(args))
     flush = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB, evicts the MALL
     for _ in range(args.warmup):
         api.nl_operator_operate_batch(op, fi, fo)
@@ -169,8 +176,7 @@ def bench_taylor(args, torch, api, _lib, workloads):
     log("building the R=12 h=0.25 sphere workload (Taylor)")
     s = workloads.hamiltonian_system()
     hm = api.HamiltonianDevice.from_system(s)
-    if args.xcd_chunk:
-        hm.kinetic.set_schedule(args.xcd_chunk)
+    hm.tune(**tuning(args))
     batches = [api.Batch.from_states(workloads.gaussian_envelope_states(s, 32, first=32 * b), s.np_part)
                for b in range(4)]
     te = api.Exponential(4)
@@ -203,11 +209,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor"])
-    ap.add_argument("--xcd-chunk", type=int, default=0)
+    ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 gather, 2 z-march")
+    ap.add_argument("--xcd-chunk", type=int, default=None)
+    ap.add_argument("--gather-tile", type=int, default=None)
+    ap.add_argument("--zseg", type=int, default=None)
+    ap.add_argument("--lane-chunk", type=int, default=None)
+    ap.add_argument("--lanes-per-thread", type=int, default=None)
     ap.add_argument("--bra", type=int, default=1)
     ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-lattice", action="store_true", help="index neighbours through ri, not the site map")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this size first (reference point for HBM counters)")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per k_apply launch measured with rocprofv3 (FETCH+WRITE, scaled)")
     args = ap.parse_args()
@@ -223,6 +237,14 @@ def main() -> int:
         return 1
     from octhip import _lib, api, workloads
     _lib.init(0)
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_calibration_mib} MiB read + written")
+        del src, dst
     fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor}[args.workload]
     res = fn(args, torch, api, _lib, workloads)
     print(json.dumps(res), flush=True)

@@ -168,12 +168,29 @@ int octhip_profiling_stop(void);
 int octhip_profiling_get(const char* region, double* total_ms, int* count);
 
 /* ------------------------------------------------------------------ tuning --
- * No counterpart in the reference (Octopus fixes launch shapes in accel.F90).  Used by the
- * autotuner; results never depend on these settings beyond summation order.
- * xcd_chunk: consecutive point tiles given to one XCD before moving to the next
- * (1 = hardware round-robin).  bra_mode: 0 = plain FMA reduction, 1 = FP64 MFMA. */
-int octhip_nl_operator_set_schedule(octhip_nl_operator_t op, int xcd_chunk);
-int octhip_hamiltonian_set_bra_mode(octhip_hamiltonian_t hm, int bra_mode);
+ * No counterpart in the reference (Octopus fixes launch shapes in accel.F90).  Key/value
+ * knobs for the autotuner; results never depend on them beyond floating-point summation
+ * order (the stencil kernels are bit-identical to each other under every setting).
+ *   KERNEL           0 automatic (z-march for the order-4 star on a lattice, gather
+ *                    otherwise), 1 gather, 2 z-march
+ *   XCD_CHUNK        gather: consecutive point tiles given to one XCD (1 = round-robin)
+ *   GATHER_TILE      gather: passes over a workgroup's point slots per tile (1..8)
+ *   ZSEG             z-march: output planes per work item (1..64)
+ *   LANE_CHUNK       z-march: lanes of a row handled by one workgroup (1..16, power of 2;
+ *                    0 = min(row length, 16))
+ *   LANES_PER_THREAD z-march: 1 or 2 doubles per thread (0 = automatic)
+ *   BRA_MODE         projector bra: 0 plain FMA reduction, 1 FP64 MFMA
+ * set returns OCTHIP_ERR_INVALID for an unknown key or a value out of range. */
+#define OCTHIP_TUNE_KERNEL            1
+#define OCTHIP_TUNE_XCD_CHUNK         2
+#define OCTHIP_TUNE_GATHER_TILE       3
+#define OCTHIP_TUNE_ZSEG              4
+#define OCTHIP_TUNE_LANE_CHUNK        5
+#define OCTHIP_TUNE_LANES_PER_THREAD  6
+#define OCTHIP_TUNE_BRA_MODE          16
+int octhip_nl_operator_set_tuning(octhip_nl_operator_t op, int key, int value);
+int octhip_nl_operator_get_tuning(octhip_nl_operator_t op, int key, int* value);
+int octhip_hamiltonian_set_tuning(octhip_hamiltonian_t hm, int key, int value);
 
 #ifdef __cplusplus
 }

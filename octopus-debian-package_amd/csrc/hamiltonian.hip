@@ -9,6 +9,7 @@ using namespace octhip;
 namespace octhip {
 void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a);
 int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s);
+int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s);
 }  // namespace octhip
 
 namespace {
@@ -19,12 +20,14 @@ int ensure_work(octhip_hamiltonian_s* h, int ldf) {
   if (h->work_ldf >= ldf) return OCTHIP_SUCCESS;
   dfree(h->d_partial);
   dfree(h->d_proj);
-  h->d_partial = h->d_proj = nullptr;
+  dfree(h->d_ksum);
+  h->d_partial = h->d_proj = h->d_ksum = nullptr;
   const size_t L = (size_t)1 << ldf;
   int rc = OCTHIP_SUCCESS;
   if (h->nmat > 0) {
     rc = dalloc((void**)&h->d_partial, sizeof(double) * h->nchunk * h->max_nprojs * L);
     if (!rc) rc = dalloc((void**)&h->d_proj, sizeof(double) * h->full_projection_size * L);
+    if (!rc) rc = dalloc((void**)&h->d_ksum, sizeof(double) * h->nslot * L);
   }
   if (rc) return rc;
   h->work_ldf = ldf;
@@ -71,7 +74,43 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s) {
   r.L = L;
   r.pjmax = h->max_nprojs;
   r.nproj = h->full_projection_size;
-  return launch_bra_reduce(r, s);
+  rc = launch_bra_reduce(r, s);
+  if (rc) return rc;
+  KetArgs k;
+  k.slot_pt = h->d_slot_pt;
+  k.pos = h->d_pos;
+  k.ent = h->d_ent;
+  k.pt = h->d_pt;
+  k.mat_pbase = h->d_mat_pbase;
+  k.mat_npj = h->d_mat_npj;
+  k.pjs = h->pjs;
+  k.proj = h->d_proj;
+  k.ksum = h->d_ksum;
+  k.nslot = h->nslot;
+  k.L = L;
+  return launch_ket(k, s);
+}
+
+// v_eff(p) = v(p) + (f0 x + f1 y + f2 z): the dipole term of laser_potential
+// (src/hamiltonian/lasers.F90:823-853) folded into the local potential once per field.
+__global__ void k_veff(double* __restrict__ veff, const double* __restrict__ v,
+                       const double* __restrict__ xyz, double f0, double f1, double f2, int np) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x)
+    veff[p] = v[p] + (f0 * xyz[3 * (size_t)p] + f1 * xyz[3 * (size_t)p + 1] + f2 * xyz[3 * (size_t)p + 2]);
+}
+
+int update_veff(octhip_hamiltonian_s* h, hipStream_t s) {
+  if (!h->has_field || !h->d_v || !h->d_xyz) return OCTHIP_SUCCESS;
+  if (!h->d_veff) {
+    int rc = dalloc((void**)&h->d_veff, sizeof(double) * h->np);
+    if (rc) return rc;
+  }
+  const int grid = std::min((h->np + BLOCK - 1) / BLOCK, 4096);
+  hipLaunchKernelGGL(k_veff, dim3(grid), dim3(BLOCK), 0, s, h->d_veff, h->d_v, h->d_xyz, h->field[0],
+                     h->field[1], h->field[2], h->np);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_veff: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
 }
 
 int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s) {
@@ -94,23 +133,11 @@ void fill_terms(const octhip_hamiltonian_s* h, int terms, ApplyArgs& a) {
     a.n = 0;  // no stencil term: runtime-size path with zero points
   }
   if ((terms & OCTHIP_TERM_LOCAL_POTENTIAL) && h->d_v) {
-    a.v = h->d_v;
-    if (h->has_field && h->d_xyz) {
-      a.xyz = h->d_xyz;
-      a.has_field = 1;
-      a.f0 = h->field[0];
-      a.f1 = h->field[1];
-      a.f2 = h->field[2];
-    }
+    a.v = (h->has_field && h->d_veff) ? h->d_veff : h->d_v;
   }
-  if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
-    a.pos = h->d_pos;
-    a.ent = h->d_ent;
-    a.pt = h->d_pt;
-    a.mat_pbase = h->d_mat_pbase;
-    a.mat_npj = h->d_mat_npj;
-    a.pjs = h->pjs;
-    a.proj = h->d_proj;
+  if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0 && h->nslot > 0) {
+    a.kslot = h->d_kslot;
+    a.ksum = h->d_ksum;
   }
 }
 
@@ -189,7 +216,16 @@ extern "C" int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double ma
       }
     }
     h->nchunk = (int)chunks.size();
-    rc = upload(&h->d_matrices, matrices, matrix_size, s);
+    std::vector<int> kslot(np, -1), slot_pt;
+    for (int p = 0; p < np; ++p)
+      if (pos[p + 1] > pos[p]) {
+        kslot[p] = (int)slot_pt.size();
+        slot_pt.push_back(p);
+      }
+    h->nslot = (int)slot_pt.size();
+    rc = upload(&h->d_kslot, kslot.data(), kslot.size(), s);
+    if (!rc) rc = upload(&h->d_slot_pt, slot_pt.data(), slot_pt.size(), s);
+    if (!rc) rc = upload(&h->d_matrices, matrices, matrix_size, s);
     if (!rc) rc = upload(&h->d_maps, maps, total_points, s);
     if (!rc) rc = upload(&h->d_scals, scals, nproj, s);
     if (!rc) rc = upload(&h->d_pt, pt.data(), pt.size(), s);
@@ -215,8 +251,12 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   (void)hipDeviceSynchronize();
   dfree(h->d_mat_info);
   dfree(h->d_proj_mat);
+  dfree(h->d_kslot);
+  dfree(h->d_slot_pt);
+  dfree(h->d_ksum);
   dfree(h->d_v);
   dfree(h->d_xyz);
+  dfree(h->d_veff);
   dfree(h->d_matrices);
   dfree(h->d_maps);
   dfree(h->d_scals);
@@ -235,11 +275,14 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   return OCTHIP_SUCCESS;
 }
 
-extern "C" int octhip_hamiltonian_set_bra_mode(octhip_hamiltonian_t h, int bra_mode) {
+extern "C" int octhip_hamiltonian_set_tuning(octhip_hamiltonian_t h, int key, int value) {
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
-  OCTHIP_REQUIRE(bra_mode == 0 || bra_mode == 1, "bra_mode must be 0 or 1");
-  h->use_mfma = bra_mode;
-  return OCTHIP_SUCCESS;
+  if (key == OCTHIP_TUNE_BRA_MODE) {
+    OCTHIP_REQUIRE(value == 0 || value == 1, "bra mode must be 0 or 1");
+    h->use_mfma = value;
+    return OCTHIP_SUCCESS;
+  }
+  return octhip_nl_operator_set_tuning(h->kin, key, value);  // stencil knobs go to the kinetic operator
 }
 
 extern "C" int octhip_hamiltonian_set_potential(octhip_hamiltonian_t h, const double* vlocal, void* stream) {
@@ -252,7 +295,7 @@ extern "C" int octhip_hamiltonian_set_potential(octhip_hamiltonian_t h, const do
   }
   OCTHIP_HIP(hipMemcpyAsync(h->d_v, vlocal, sizeof(double) * h->np, hipMemcpyHostToDevice, s));
   OCTHIP_HIP(hipStreamSynchronize(s));
-  return OCTHIP_SUCCESS;
+  return update_veff(h, s);
 }
 
 extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const double* positions,
@@ -262,6 +305,7 @@ extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const
   hipStream_t s = (hipStream_t)stream;
   if (positions) {
     dfree(h->d_xyz);
+  dfree(h->d_veff);
     h->d_xyz = nullptr;
     int rc = upload(&h->d_xyz, positions, 3 * (size_t)h->np, s);
     if (rc) return rc;
@@ -269,7 +313,7 @@ extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const
   OCTHIP_REQUIRE(h->d_xyz != nullptr, "set_dipole_field: positions never uploaded");
   for (int d = 0; d < 3; ++d) h->field[d] = field[d];
   h->has_field = true;
-  return OCTHIP_SUCCESS;
+  return update_veff(h, s);
 }
 
 extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
@@ -299,7 +343,7 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   a.fo = hpsi;
   a.L = L;
   a.lim = set_bc ? op->np : op->np_part;
-  return launch_apply(a, op->lattice && a.n > 0, EPI_STORE, s);
+  return launch_operator(h->kin, a, EPI_STORE, s);
 }
 
 extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
@@ -337,7 +381,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
     a.lim = op->np;  // boundary rows are zero: psi's were just cleared, work buffers have none
     if (order == 1) {
       a.fo = h->d_y[0];
-      rc = launch_apply(a, op->lattice, EPI_STORE, s);
+      rc = launch_operator(h->kin, a, EPI_STORE, s);
       if (rc) return rc;
       const size_t count = (size_t)op->np * L;
       size_t grid = std::min<size_t>((count + BLOCK - 1) / BLOCK, 8192);
@@ -358,7 +402,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
       a.c2r = z.real();
       a.c2i = z.imag();
     }
-    rc = launch_apply(a, op->lattice, EPI_TAYLOR, s);
+    rc = launch_operator(h->kin, a, EPI_TAYLOR, s);
     if (rc) return rc;
   }
   return OCTHIP_SUCCESS;

@@ -21,29 +21,25 @@ struct ApplyArgs {
   int np;                          // rows computed
   int lim;                         // neighbour rows >= lim read as zero
   int L;                           // doubles per row
-  int tpp, tshift, ppb;            // threads per point, log2(tpp), points per block
+  int tpp, tshift;                 // threads per point (power of 2), log2(tpp)
+  int tpts;                        // points per tile (one workgroup)
+  int sstride;                     // LDS row stride of the tile's neighbour table
   int chunk;                       // blockIdx -> tile remap granularity (XCD locality)
+  int tile_iters;                  // requested passes per tile (0 = default)
   int ntiles;
   int n;                           // stencil size (runtime-size path)
   const int* __restrict__ cidx;    // lattice: site of point p
   const int* __restrict__ cube;    // lattice: site -> point
+  const int* __restrict__ dcp;     // lattice: site delta of stencil point j (device copy)
+  int dc[MAX_STENCIL];             // lattice: the same deltas as kernel arguments
   const int* __restrict__ ri;      // run-compressed offsets
   const int* __restrict__ rimap;   // run of point p
-  int dc[MAX_STENCIL];             // lattice: site delta of stencil point j
   double w[MAX_STENCIL];           // weights already multiplied by the factor
-  // local potential: v_eff = v + f . x
+  // local potential (the dipole term, if any, is already folded in)
   const double* __restrict__ v;
-  const double* __restrict__ xyz;
-  double f0, f1, f2;
-  int has_field;
-  // nonlocal ket: per point, entries (global sphere point, matrix) in matrix order
-  const int* __restrict__ pos;
-  const int2* __restrict__ ent;
-  const double* __restrict__ pt;   // transposed projector rows, stride pjs
-  const int* __restrict__ mat_pbase;
-  const int* __restrict__ mat_npj;
-  int pjs;
-  const double* __restrict__ proj; // projections (full_projection_size x L)
+  // nonlocal ket: kslot[p] = row of ksum holding sum_a P_a proj_a at point p, or -1
+  const int* __restrict__ kslot;
+  const double* __restrict__ ksum;
   // Taylor epilogue: yout = Hy ; acc += c1 * y + c2 * Hy (complex lanes)
   double* yout;
   double* acc;
@@ -64,92 +60,150 @@ __device__ __forceinline__ double zmul_lane(double zr, double zi, double x, doub
   return odd ? (zr * x + zi * xq) : (zr * x - zi * xq);
 }
 
-template <int N, int IDX, bool UNI, int EPI>
+template <int LPT>
+struct Vec;
+template <>
+struct Vec<1> {
+  double v[1];
+};
+template <>
+struct Vec<2> {
+  double v[2];
+};
+
+template <int LPT>
+__device__ __forceinline__ Vec<LPT> ldv(const double* p) {
+  Vec<LPT> r;
+  if constexpr (LPT == 2) {
+    const double2 t = *reinterpret_cast<const double2*>(p);
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+  } else {
+    r.v[0] = *p;
+  }
+  return r;
+}
+
+template <int LPT>
+__device__ __forceinline__ void stv(double* p, const Vec<LPT>& r) {
+  if constexpr (LPT == 2) {
+    *reinterpret_cast<double2*>(p) = make_double2(r.v[0], r.v[1]);
+  } else {
+    *p = r.v[0];
+  }
+}
+
+// Fused apply over a tile of `tpts` consecutive points per workgroup.
+//  phase 1: the tile's neighbour indices (site map or run-compressed ri) are gathered
+//           into LDS by all threads with vector loads -- one memory round trip per tile;
+//  phase 2: threads stream rows; a thread owns LPT consecutive doubles of one point (one
+//           complex value for LPT == 2), tpp threads cover a point's row, BLOCK/tpp points
+//           are processed per pass.  Per lane: stencil sum in stencil order with fma,
+//           then fma(v, psi), then the KB ket terms matrix by matrix, then the epilogue.
+template <int N, int IDX, int EPI, int LPT>
 __global__ __launch_bounds__(BLOCK) void k_apply(const ApplyArgs a) {
+  extern __shared__ int sidx[];
   const int tile = tile_of(blockIdx.x, a.chunk);
   if (tile >= a.ntiles) return;
-  int lp, lane0, tpp;
-  if constexpr (UNI) {
-    lp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    lane0 = threadIdx.x & 63;
-    tpp = 64;
-  } else {
-    lp = threadIdx.x >> a.tshift;
-    lane0 = threadIdx.x & (a.tpp - 1);
-    tpp = a.tpp;
-  }
-  const int p = tile * a.ppb + lp;
-  if (p >= a.np) return;
-  const size_t L = (size_t)a.L;
-  constexpr int NQ = N > 0 ? N : 1;
-  int q[NQ];
+  const int n = N > 0 ? N : a.n;
+  const int ss = a.sstride;
+  const int p0 = tile * a.tpts;
+  const int cnt = min(a.tpts, a.np - p0);
   if constexpr (N > 0) {
-    if constexpr (IDX == IDX_LATTICE) {
-      const int c = a.cidx[p];
+    // one thread per tile point: its N index loads are independent and issued together
+    for (int t = threadIdx.x; t < cnt; t += BLOCK) {
+      const int p = p0 + t;
+      int q[N];
+      if constexpr (IDX == IDX_LATTICE) {
+        const int c = a.cidx[p];
 #pragma unroll
-      for (int j = 0; j < N; ++j) q[j] = a.cube[c + a.dc[j]];
-    } else {
-      const int* row = a.ri + (size_t)a.rimap[p] * N;
+        for (int j = 0; j < N; ++j) q[j] = a.cube[c + a.dc[j]];
+      } else {
+        const int* row = a.ri + (size_t)a.rimap[p] * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) q[j] = p + row[j];
-    }
-  }
-  double vv = 0.0;
-  if (a.v) {
-    vv = a.v[p];
-    if (a.has_field)
-      vv = vv + (a.f0 * a.xyz[3 * (size_t)p] + a.f1 * a.xyz[3 * (size_t)p + 1] +
-                 a.f2 * a.xyz[3 * (size_t)p + 2]);
-  }
-  int e0 = 0, e1 = 0;
-  if (a.pos) {
-    e0 = a.pos[p];
-    e1 = a.pos[p + 1];
-  }
-  const double* own = a.fi + (size_t)p * L;
-  for (int l = lane0; l < a.L; l += tpp) {
-    double s = 0.0;
-    if constexpr (N > 0) {
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const int qq = q[j];
-        const double* src = ((unsigned)qq < (unsigned)a.lim) ? a.fi + (size_t)qq * L : a.zrow;
-        s = fma(a.w[j], src[l], s);
+        for (int j = 0; j < N; ++j) q[j] = p + row[j];
       }
-    } else {
-      const int c = (IDX == IDX_LATTICE) ? a.cidx[p] : 0;
-      const int* row = (IDX == IDX_RI) ? a.ri + (size_t)a.rimap[p] * a.n : nullptr;
-      for (int j = 0; j < a.n; ++j) {
-        const int qq = (IDX == IDX_LATTICE) ? a.cube[c + a.dc[j]] : p + row[j];
-        const double* src = ((unsigned)qq < (unsigned)a.lim) ? a.fi + (size_t)qq * L : a.zrow;
-        s = fma(a.w[j], src[l], s);
+#pragma unroll
+      for (int j = 0; j < N; ++j) sidx[t * ss + j] = q[j];
+    }
+    __syncthreads();
+  } else if (n > 0) {
+    for (int e = threadIdx.x; e < cnt * n; e += BLOCK) {
+      const int t = e / n;
+      const int j = e - t * n;
+      const int p = p0 + t;
+      int q;
+      if constexpr (IDX == IDX_LATTICE) {
+        q = a.cube[a.cidx[p] + a.dcp[j]];
+      } else {
+        q = p + a.ri[(size_t)a.rimap[p] * n + j];
       }
+      sidx[t * ss + j] = q;
     }
-    const double x = own[l];
-    if (a.v) s = fma(vv, x, s);
-    for (int e = e0; e < e1; ++e) {
-      const int2 en = a.ent[e];
-      const int pb = a.mat_pbase[en.y];
-      const int nj = a.mat_npj[en.y];
-      const double* prow = a.pt + (size_t)en.x * a.pjs;
-      double t = 0.0;
-      for (int jj = 0; jj < nj; ++jj) t = fma(prow[jj], a.proj[(size_t)(pb + jj) * L + l], t);
-      s = s + t;
-    }
-    if constexpr (EPI == EPI_STORE) {
-      a.fo[(size_t)p * L + l] = s;
-    } else {
-      if (a.yout) a.yout[(size_t)p * L + l] = s;
-      if (a.acc) {
-        const bool odd = (l & 1) != 0;
-        double av = a.acc[(size_t)p * L + l];
-        if (a.has_c1) {
-          const double xq = __shfl_xor(x, 1);
-          av = av + zmul_lane(a.c1r, a.c1i, x, xq, odd);
+    __syncthreads();
+  }
+  const int tpp = a.tpp;
+  const int slots = BLOCK >> a.tshift;
+  const int slot = threadIdx.x >> a.tshift;
+  const int lane0 = (threadIdx.x & (tpp - 1)) * LPT;
+  const size_t L = (size_t)a.L;
+  for (int t = slot; t < cnt; t += slots) {
+    const int p = p0 + t;
+    const int* qi = sidx + t * ss;
+    const double vv = a.v ? a.v[p] : 0.0;
+    const int ks = a.kslot ? a.kslot[p] : -1;
+    for (int l = lane0; l < a.L; l += tpp * LPT) {
+      Vec<LPT> s;
+#pragma unroll
+      for (int k = 0; k < LPT; ++k) s.v[k] = 0.0;
+      if constexpr (N > 0) {
+        int q[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) q[j] = qi[j];
+        Vec<LPT> x[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const double* src = ((unsigned)q[j] < (unsigned)a.lim) ? a.fi + (size_t)q[j] * L : a.zrow;
+          x[j] = ldv<LPT>(src + l);
         }
-        const double sq = __shfl_xor(s, 1);
-        av = av + zmul_lane(a.c2r, a.c2i, s, sq, odd);
-        a.acc[(size_t)p * L + l] = av;
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+          for (int k = 0; k < LPT; ++k) s.v[k] = fma(a.w[j], x[j].v[k], s.v[k]);
+      } else {
+        for (int j = 0; j < n; ++j) {
+          const int qq = qi[j];
+          const double* src = ((unsigned)qq < (unsigned)a.lim) ? a.fi + (size_t)qq * L : a.zrow;
+          const Vec<LPT> x = ldv<LPT>(src + l);
+#pragma unroll
+          for (int k = 0; k < LPT; ++k) s.v[k] = fma(a.w[j], x.v[k], s.v[k]);
+        }
+      }
+      const Vec<LPT> x = ldv<LPT>(a.fi + (size_t)p * L + l);
+      if (a.v) {
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) s.v[k] = fma(vv, x.v[k], s.v[k]);
+      }
+      if (ks >= 0) {
+        const Vec<LPT> kv = ldv<LPT>(a.ksum + (size_t)ks * L + l);
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kv.v[k];
+      }
+      if constexpr (EPI == EPI_STORE) {
+        stv<LPT>(a.fo + (size_t)p * L + l, s);
+      } else {
+        static_assert(EPI != EPI_TAYLOR || LPT == 2, "Taylor epilogue needs complex lane pairs");
+        if (a.yout) stv<LPT>(a.yout + (size_t)p * L + l, s);
+        if (a.acc) {
+          Vec<LPT> av = ldv<LPT>(a.acc + (size_t)p * L + l);
+          if (a.has_c1) {
+            av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], x.v[1], false);
+            av.v[1] = av.v[1] + zmul_lane(a.c1r, a.c1i, x.v[1], x.v[0], true);
+          }
+          av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], s.v[1], false);
+          av.v[1] = av.v[1] + zmul_lane(a.c2r, a.c2i, s.v[1], s.v[0], true);
+          stv<LPT>(a.acc + (size_t)p * L + l, av);
+        }
       }
     }
   }
@@ -198,7 +252,24 @@ struct BraReduceArgs {
   int L, pjmax, nproj;
 };
 
+// ksum[slot, l] = sum over the point's (sphere point, matrix) entries, in matrix order, of
+// sum_j P[sphere point, j] proj[matrix base + j, l]  (the ket of X(hamiltonian_elec_base_
+// nlocal_finish), src/hamiltonian/hamiltonian_elec_base_inc.F90:914-943, per mesh point)
+struct KetArgs {
+  const int* __restrict__ slot_pt;   // slot -> mesh point
+  const int* __restrict__ pos;       // CSR of entries per mesh point
+  const int2* __restrict__ ent;      // (global sphere point, matrix)
+  const double* __restrict__ pt;     // transposed projector rows, stride pjs
+  const int* __restrict__ mat_pbase;
+  const int* __restrict__ mat_npj;
+  int pjs;
+  const double* __restrict__ proj;
+  double* __restrict__ ksum;
+  int nslot, L;
+};
+
 int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s);
+int launch_ket(const KetArgs& k, hipStream_t s);
 int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s);
 
 // sphere points per bra chunk

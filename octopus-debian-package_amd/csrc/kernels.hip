@@ -1,54 +1,61 @@
 // Kernel instances and launchers: fused apply, projector bra (FP64 MFMA and plain).
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace octhip {
 
 namespace {
 
-template <int N, int IDX, int EPI>
+template <int N, int IDX, int EPI, int LPT>
 int launch_apply_n(ApplyArgs& a, hipStream_t s) {
-  const int tiles = a.ntiles;
   const int per = 8 * (a.chunk > 1 ? a.chunk : 1);
-  const int grid = ((tiles + per - 1) / per) * per;
+  const int grid = ((a.ntiles + per - 1) / per) * per;
+  const size_t shmem = (size_t)a.tpts * a.sstride * sizeof(int);
   ProfScope prof("k_apply", s);
-  if (a.tpp == 64)
-    hipLaunchKernelGGL((k_apply<N, IDX, true, EPI>), dim3(grid), dim3(BLOCK), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_apply<N, IDX, false, EPI>), dim3(grid), dim3(BLOCK), 0, s, a);
+  hipLaunchKernelGGL((k_apply<N, IDX, EPI, LPT>), dim3(grid), dim3(BLOCK), shmem, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_apply launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
 
-template <int IDX, int EPI>
-int launch_apply_idx(ApplyArgs& a, hipStream_t s) {
+template <int IDX, int EPI, int LPT>
+int launch_apply_lpt(ApplyArgs& a, hipStream_t s) {
   switch (a.n) {
-    case 25: return launch_apply_n<25, IDX, EPI>(a, s);
-    case 19: return launch_apply_n<19, IDX, EPI>(a, s);
-    case 13: return launch_apply_n<13, IDX, EPI>(a, s);
-    case 7: return launch_apply_n<7, IDX, EPI>(a, s);
-    default: return launch_apply_n<0, IDX, EPI>(a, s);
+    case 25: return launch_apply_n<25, IDX, EPI, LPT>(a, s);
+    case 19: return launch_apply_n<19, IDX, EPI, LPT>(a, s);
+    case 13: return launch_apply_n<13, IDX, EPI, LPT>(a, s);
+    case 7: return launch_apply_n<7, IDX, EPI, LPT>(a, s);
+    default: return launch_apply_n<0, IDX, EPI, LPT>(a, s);
   }
+}
+
+template <int IDX>
+int launch_apply_idx(ApplyArgs& a, int epi, int lpt, hipStream_t s) {
+  if (epi == EPI_TAYLOR) return launch_apply_lpt<IDX, EPI_TAYLOR, 2>(a, s);
+  return lpt == 2 ? launch_apply_lpt<IDX, EPI_STORE, 2>(a, s) : launch_apply_lpt<IDX, EPI_STORE, 1>(a, s);
 }
 
 }  // namespace
 
 int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s) {
   if (a.np <= 0) return OCTHIP_SUCCESS;
-  int tpp = a.L < 64 ? a.L : 64;
+  const int lpt = a.L >= 2 ? 2 : 1;
+  if (epi == EPI_TAYLOR && lpt != 2) return fail(OCTHIP_ERR_INVALID, "Taylor epilogue needs complex rows");
+  int tpp = a.L / lpt;
+  if (tpp > 64) tpp = 64;
   int tshift = 0;
   while ((1 << tshift) < tpp) ++tshift;
   a.tpp = tpp;
   a.tshift = tshift;
-  a.ppb = BLOCK / tpp;
-  a.ntiles = (a.np + a.ppb - 1) / a.ppb;
+  const int slots = BLOCK / tpp;
+  int iters = a.tile_iters > 0 ? a.tile_iters : 4;
+  while (iters > 1 && slots * iters > 256) --iters;
+  a.tpts = slots * iters;
+  a.sstride = a.n > 0 ? ((a.n + 3) & ~3) : 1;
+  a.ntiles = (a.np + a.tpts - 1) / a.tpts;
   if (a.chunk < 1) a.chunk = 1;
-  if (lattice) {
-    return epi == EPI_STORE ? launch_apply_idx<IDX_LATTICE, EPI_STORE>(a, s)
-                            : launch_apply_idx<IDX_LATTICE, EPI_TAYLOR>(a, s);
-  }
-  return epi == EPI_STORE ? launch_apply_idx<IDX_RI, EPI_STORE>(a, s)
-                          : launch_apply_idx<IDX_RI, EPI_TAYLOR>(a, s);
+  return lattice ? launch_apply_idx<IDX_LATTICE>(a, epi, lpt, s) : launch_apply_idx<IDX_RI>(a, epi, lpt, s);
 }
 
 // -------------------------------------------------------------- bra (plain) --
@@ -204,6 +211,38 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
   hipLaunchKernelGGL(k_bra_mfma, dim3(nchunk), dim3(BLOCK), 0, s, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra_mfma launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
+__global__ void k_ket(const KetArgs k) {
+  const size_t total = (size_t)k.nslot * k.L;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int sl = (int)(t / k.L);
+    const int l = (int)(t - (size_t)sl * k.L);
+    const int p = k.slot_pt[sl];
+    double s = 0.0;
+    for (int e = k.pos[p]; e < k.pos[p + 1]; ++e) {
+      const int2 en = k.ent[e];
+      const int pb = k.mat_pbase[en.y];
+      const int nj = k.mat_npj[en.y];
+      const double* prow = k.pt + (size_t)en.x * k.pjs;
+      double t2 = 0.0;
+      for (int jj = 0; jj < nj; ++jj) t2 = fma(prow[jj], k.proj[(size_t)(pb + jj) * k.L + l], t2);
+      s = s + t2;
+    }
+    k.ksum[t] = s;
+  }
+}
+
+int launch_ket(const KetArgs& k, hipStream_t s) {
+  const size_t total = (size_t)k.nslot * k.L;
+  if (total == 0) return OCTHIP_SUCCESS;
+  int grid = (int)std::min<size_t>((total + BLOCK - 1) / BLOCK, 16384);
+  ProfScope prof("k_ket", s);
+  hipLaunchKernelGGL(k_ket, dim3(grid), dim3(BLOCK), 0, s, k);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_ket launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
 

@@ -4,7 +4,9 @@
 #include <cstring>
 #include <deque>
 
-#include "kernels.h"
+#include <map>
+
+#include "zmarch.h"
 
 using namespace octhip;
 
@@ -91,11 +93,154 @@ int build_site_map(octhip_nl_operator_s* op, const int* pts, hipStream_t s) {
     op->dcube[j] = (int)(pts[3 * j] + sz[0] * (pts[3 * j + 1] + sz[1] * (long)pts[3 * j + 2]));
   op->ncube = (size_t)nsite;
   int rc = upload(&op->d_cube, cube.data(), cube.size(), s);
+  if (!rc) rc = upload(&op->d_dcube, op->dcube, (size_t)n, s);
   if (rc) return rc;
   return upload(&op->d_cidx, cidx.data(), cidx.size(), s);
 }
 
+// The z-march kernel needs the order-4 star in Octopus's point order: x -4..4, then
+// y -4..-1,1..4, then z -4..-1,1..4 (src/grid/stencil_star.F90:59-93).
+bool is_star4(int n, const int* pts) {
+  if (n != 25) return false;
+  int j = 0;
+  for (int d = 0; d < 3; ++d)
+    for (int k = -4; k <= 4; ++k) {
+      if (k == 0 && d > 0) continue;
+      for (int e = 0; e < 3; ++e)
+        if (pts[3 * j + e] != (e == d ? k : 0)) return false;
+      ++j;
+    }
+  return true;
+}
+
+// March cube: the inner points' bounding box rounded up to whole 8x8 column tiles in x
+// and y, padded by the stencil reach in x/y and by ZPADZ planes in z; site -> point index.
+int build_march(octhip_nl_operator_s* op, hipStream_t s) {
+  const int np = op->np, np_part = op->np_part;
+  const std::vector<int>& c = op->coords;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (int q = 0; q < np; ++q)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], c[3 * (size_t)q + d]);
+      hi[d] = std::max(hi[d], c[3 * (size_t)q + d]);
+    }
+  op->ntx = (hi[0] - lo[0] + ZT) / ZT;
+  op->nty = (hi[1] - lo[1] + ZT) / ZT;
+  op->mx = op->ntx * ZT + 2 * ZR;
+  op->my = op->nty * ZT + 2 * ZR;
+  op->mz = (hi[2] - lo[2] + 1) + 2 * ZPADZ;
+  const int ox = lo[0] - ZR, oy = lo[1] - ZR, oz = lo[2] - ZPADZ;
+  const size_t nsite = (size_t)op->mx * op->my * op->mz;
+  if (nsite > (size_t)INT_MAX / 2) return fail(OCTHIP_ERR_UNSUPPORTED, "march cube too large");
+  std::vector<int> mc(nsite, -1);
+  op->tile_zlo.assign((size_t)op->ntx * op->nty, INT_MAX);
+  op->tile_zhi.assign((size_t)op->ntx * op->nty, -1);
+  for (int q = 0; q < np_part; ++q) {
+    const int* p = &c[3 * (size_t)q];
+    if (p[0] == INT_MIN) continue;
+    const long x = p[0] - ox, y = p[1] - oy, z = p[2] - oz;
+    if (x < 0 || y < 0 || z < 0 || x >= op->mx || y >= op->my || z >= op->mz) continue;
+    mc[(size_t)x + (size_t)op->mx * ((size_t)y + (size_t)op->my * z)] = q;
+    if (q < np) {
+      const int t = (int)((x - ZR) / ZT + op->ntx * ((y - ZR) / ZT));
+      op->tile_zlo[t] = std::min(op->tile_zlo[t], (int)z);
+      op->tile_zhi[t] = std::max(op->tile_zhi[t], (int)z);
+    }
+  }
+  return upload(&op->d_mcube, mc.data(), mc.size(), s);
+}
+
 }  // namespace
+
+namespace octhip {
+// (tile, z-segment) work items for the current segment length
+int zmarch_items(octhip_nl_operator_s* op) {
+  if (op->zseg_built == op->zseg && op->d_items) return OCTHIP_SUCCESS;
+  std::vector<int4> items;
+  // segment-major, then tile rows: concurrently running items share halo rows in L2
+  int maxlen = 0;
+  for (size_t t = 0; t < op->tile_zlo.size(); ++t)
+    if (op->tile_zhi[t] >= 0) maxlen = std::max(maxlen, op->tile_zhi[t] - op->tile_zlo[t] + 1);
+  for (int sg = 0; sg * op->zseg < maxlen; ++sg)
+    for (int ty = 0; ty < op->nty; ++ty)
+      for (int tx = 0; tx < op->ntx; ++tx) {
+        const int t = tx + op->ntx * ty;
+        if (op->tile_zhi[t] < 0) continue;
+        const int zb = op->tile_zlo[t] + sg * op->zseg;
+        if (zb > op->tile_zhi[t]) continue;
+        const int ze = std::min(zb + op->zseg, op->tile_zhi[t] + 1);
+        items.push_back(make_int4(tx, ty, zb, ze));
+      }
+  dfree(op->d_items);
+  op->d_items = nullptr;
+  for (auto& e : op->sched) dfree(e.second);
+  op->sched.clear();
+  op->sched_len.clear();
+  op->items = items;
+  op->nitems = (int)items.size();
+  int rc = upload(&op->d_items, items.data(), items.size(), nullptr);
+  if (rc) return rc;
+  op->zseg_built = op->zseg;
+  return OCTHIP_SUCCESS;
+}
+
+// blockIdx -> (item, lane chunk): the (chunk, item) list is cut into 8 contiguous parts,
+// part k runs on XCD k (workgroups are dealt to XCDs round-robin by blockIdx).
+int zmarch_sched(octhip_nl_operator_s* op, int nchunk, const int2** out, int* len) {
+  for (size_t i = 0; i < op->sched.size(); ++i)
+    if (op->sched[i].first == nchunk) {
+      *out = op->sched[i].second;
+      *len = op->sched_len[i];
+      return OCTHIP_SUCCESS;
+    }
+  const int n = op->nitems * nchunk;
+  const int per = (n + 7) / 8;
+  std::vector<int2> sc((size_t)per * 8, make_int2(-1, 0));
+  for (int k = 0; k < 8; ++k)
+    for (int i = 0; i < per; ++i) {
+      const int g = k * per + i;
+      if (g >= n) break;
+      sc[(size_t)i * 8 + k] = make_int2(g % op->nitems, g / op->nitems);
+    }
+  int2* d = nullptr;
+  int rc = upload(&d, sc.data(), sc.size(), nullptr);
+  if (rc) return rc;
+  op->sched.push_back({nchunk, d});
+  op->sched_len.push_back((int)sc.size());
+  *out = d;
+  *len = (int)sc.size();
+  return OCTHIP_SUCCESS;
+}
+
+int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, hipStream_t s);
+
+// Apply through the best kernel for this operator: z-march for the order-4 star on a
+// lattice, the tiled gather otherwise.
+int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s) {
+  const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25;
+  if (!zm) return launch_apply(a, op->lattice && a.n > 0, epi, s);
+  int rc = zmarch_items(op);
+  if (rc) return rc;
+  if (op->nitems == 0 || a.np <= 0) return OCTHIP_SUCCESS;
+  int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
+  if (lc > a.L) lc = a.L;
+  int lpt = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
+  if (lpt > lc) lpt = lc;
+  if (lc == 32) lpt = 2;  // 1024 threads per workgroup at most
+  const int2* sched = nullptr;
+  int len = 0;
+  rc = zmarch_sched(op, a.L / lc, &sched, &len);
+  if (rc) return rc;
+  ZArgs z;
+  z.a = a;
+  z.mcube = op->d_mcube;
+  z.mx = op->mx;
+  z.my = op->my;
+  z.items = op->d_items;
+  z.sched = sched;
+  return launch_zmarch(z, lc, lpt, epi, len, op->zseg_built, s);
+}
+}  // namespace octhip
 
 extern "C" int octhip_nl_operator_create(int n, const double* w, const int* stencil_points, int nri,
                                          const int* ri, const int* rimap_inv, int np, int np_part,
@@ -152,6 +297,10 @@ extern "C" int octhip_nl_operator_create(int n, const double* w, const int* sten
     if (ok) {
       rc = build_site_map(op, stencil_points, s);
       if (!rc) op->lattice = true;
+      if (!rc && is_star4(n, stencil_points)) {
+        rc = build_march(op, s);
+        if (!rc) op->zmarch_ok = true;
+      }
     }
   }
   if (rc) {
@@ -166,16 +315,62 @@ extern "C" int octhip_nl_operator_destroy(octhip_nl_operator_t op) {
   if (!op) return OCTHIP_SUCCESS;
   dfree(op->d_cidx);
   dfree(op->d_cube);
+  dfree(op->d_dcube);
+  dfree(op->d_mcube);
+  dfree(op->d_items);
+  for (auto& e : op->sched) dfree(e.second);
   dfree(op->d_ri);
   dfree(op->d_rimap);
   delete op;
   return OCTHIP_SUCCESS;
 }
 
-extern "C" int octhip_nl_operator_set_schedule(octhip_nl_operator_t op, int xcd_chunk) {
+extern "C" int octhip_nl_operator_set_tuning(octhip_nl_operator_t op, int key, int value) {
   OCTHIP_REQUIRE(op != nullptr, "null operator handle");
-  OCTHIP_REQUIRE(xcd_chunk >= 1 && xcd_chunk <= 4096, "xcd_chunk %d outside 1..4096", xcd_chunk);
-  op->xcd_chunk = xcd_chunk;
+  switch (key) {
+    case OCTHIP_TUNE_KERNEL:
+      OCTHIP_REQUIRE(value >= 0 && value <= 2, "kernel %d outside 0..2", value);
+      OCTHIP_REQUIRE(value != 2 || op->zmarch_ok, "z-march needs the order-4 star on a lattice");
+      op->kernel = value;
+      return OCTHIP_SUCCESS;
+    case OCTHIP_TUNE_XCD_CHUNK:
+      OCTHIP_REQUIRE(value >= 1 && value <= 4096, "xcd_chunk %d outside 1..4096", value);
+      op->xcd_chunk = value;
+      return OCTHIP_SUCCESS;
+    case OCTHIP_TUNE_GATHER_TILE:
+      OCTHIP_REQUIRE(value >= 1 && value <= 8, "gather tile %d outside 1..8", value);
+      op->tile_iters = value;
+      return OCTHIP_SUCCESS;
+    case OCTHIP_TUNE_ZSEG:
+      OCTHIP_REQUIRE(value >= 1 && value <= 64, "z segment %d outside 1..64", value);
+      op->zseg = value;
+      return OCTHIP_SUCCESS;
+    case OCTHIP_TUNE_LANE_CHUNK:
+      OCTHIP_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4 || value == 8 || value == 16 ||
+                         value == 32,
+                     "lane chunk %d not in {0,1,2,4,8,16,32}", value);
+      op->lane_chunk = value;
+      return OCTHIP_SUCCESS;
+    case OCTHIP_TUNE_LANES_PER_THREAD:
+      OCTHIP_REQUIRE(value >= 0 && value <= 2, "lanes per thread %d outside 0..2", value);
+      op->lanes_per_thread = value;
+      return OCTHIP_SUCCESS;
+    default:
+      return fail(OCTHIP_ERR_INVALID, "unknown operator tuning key %d", key);
+  }
+}
+
+extern "C" int octhip_nl_operator_get_tuning(octhip_nl_operator_t op, int key, int* value) {
+  OCTHIP_REQUIRE(op != nullptr && value != nullptr, "null argument");
+  switch (key) {
+    case OCTHIP_TUNE_KERNEL: *value = op->kernel; break;
+    case OCTHIP_TUNE_XCD_CHUNK: *value = op->xcd_chunk; break;
+    case OCTHIP_TUNE_GATHER_TILE: *value = op->tile_iters; break;
+    case OCTHIP_TUNE_ZSEG: *value = op->zseg; break;
+    case OCTHIP_TUNE_LANE_CHUNK: *value = op->lane_chunk; break;
+    case OCTHIP_TUNE_LANES_PER_THREAD: *value = op->lanes_per_thread; break;
+    default: return fail(OCTHIP_ERR_INVALID, "unknown operator tuning key %d", key);
+  }
   return OCTHIP_SUCCESS;
 }
 
@@ -192,6 +387,7 @@ void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a) {
   a.np = op->np;
   a.cidx = op->d_cidx;
   a.cube = op->d_cube;
+  a.dcp = op->d_dcube;
   a.ri = op->d_ri;
   a.rimap = op->d_rimap;
   for (int j = 0; j < op->n; ++j) {
@@ -200,6 +396,7 @@ void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a) {
   }
   a.zrow = zero_row();
   a.chunk = op->xcd_chunk;
+  a.tile_iters = op->tile_iters;
 }
 }  // namespace octhip
 
@@ -219,7 +416,7 @@ extern "C" int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const d
   a.fo = fo;
   a.L = 1 << ldf;
   a.lim = op->np_part;  // boundary rows are read as the caller set them
-  return launch_apply(a, op->lattice, EPI_STORE, (hipStream_t)stream);
+  return launch_operator(op, a, EPI_STORE, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------- stateless leaf --

@@ -85,13 +85,31 @@ struct octhip_nl_operator_s {
   int dcube[octhip::MAX_STENCIL] = {0};  // site-index delta of stencil point j
   int* d_cidx = nullptr;                 // np: site index of each inner point
   int* d_cube = nullptr;                 // sites: point index or -1
+  int* d_dcube = nullptr;                // device copy of dcube
   size_t ncube = 0;
   // run-compressed indexing (Octopus's own ri/rimap)
   int* d_ri = nullptr;                   // nri*n
   int* d_rimap = nullptr;                // np
   // host copy of lattice coordinates (3*np_part), kept for schedules
   std::vector<int> coords;
-  int xcd_chunk = 1;                     // blockIdx -> tile remap granularity
+  int xcd_chunk = 32;                    // gather kernel: blockIdx -> tile remap granularity
+  int tile_iters = 4;                    // gather kernel: passes over the point slots per tile
+  // z-march (order-4 star on a lattice): padded march cube and (tile, z-segment) items
+  int kernel = 0;                        // 0 = auto, 1 = gather, 2 = z-march
+  bool zmarch_ok = false;
+  int* d_mcube = nullptr;
+  int mx = 0, my = 0, mz = 0;
+  int zseg = 16;                         // output planes per work item
+  int lane_chunk = 0;                    // lanes per workgroup (0 = automatic)
+  int lanes_per_thread = 0;              // doubles per thread (0 = automatic)
+  int zseg_built = 0;
+  std::vector<int> tile_zlo, tile_zhi;   // per tile: inner plane range (cube z), -1 if empty
+  int ntx = 0, nty = 0;
+  int4* d_items = nullptr;
+  int nitems = 0;
+  std::vector<int4> items;
+  std::vector<std::pair<int, int2*>> sched;  // lane chunks -> device schedule
+  std::vector<int> sched_len;
 };
 
 struct octhip_hamiltonian_s {
@@ -100,6 +118,7 @@ struct octhip_hamiltonian_s {
   int np = 0;
   double* d_v = nullptr;        // np local potential
   double* d_xyz = nullptr;      // 3*np positions for the dipole term (optional)
+  double* d_veff = nullptr;     // np: v + field . x when a dipole field is set
   double field[3] = {0, 0, 0};
   bool has_field = false;
   // projectors
@@ -123,6 +142,10 @@ struct octhip_hamiltonian_s {
   int* d_mat_chunk0 = nullptr;  // nmat+1 first chunk of each matrix
   int4* d_mat_info = nullptr;   // nmat: {npoints, nprojs, matrix offset, map offset}
   int* d_proj_mat = nullptr;    // projection index -> matrix
+  int nslot = 0;                // mesh points inside at least one sphere
+  int* d_kslot = nullptr;       // np: slot of each mesh point or -1
+  int* d_slot_pt = nullptr;     // nslot: mesh point of each slot
+  double* d_ksum = nullptr;     // nslot * L ket sums
   // work buffers sized for the largest ldf seen
   int work_ldf = -1;
   double* d_partial = nullptr;  // nchunk * max_nprojs * L

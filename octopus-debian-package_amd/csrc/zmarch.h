@@ -1,0 +1,296 @@
+// 2.5-D "z-march" apply for the order-4 star stencil on a lattice (gfx950).
+//
+// The gather kernel (k_apply) reads 25 full rows per point, so L2 has to deliver 25x the
+// HBM traffic.  Here a workgroup owns an 8x8 tile of lattice columns and one chunk of LC
+// lanes, and walks the tile's z range plane by plane:
+//   * the current plane (tile + 4-site halo strips on the four sides) is staged in LDS
+//     (double-buffered): x and y neighbours are LDS reads;
+//   * each thread keeps its column's values at z-4..z+4 in registers: z neighbours cost
+//     nothing;
+//   * loads run two planes ahead (halo values), six ahead (own column values) and one
+//     more for the site -> point indices, into ring buffers with compile-time slots (the
+//     z loop is unrolled by the ring period), so a step waits only on old loads.
+// Per output point the tile reads 1 own row chunk + 2 halo chunks (128 halo sites for 64
+// columns) instead of 25 rows.  The lattice is a padded "march cube" of point indices;
+// sites that are not points hold -1.  Values follow the gather kernel exactly: point
+// q < lim -> its row, anything else -> 0, and the 25 terms are summed in stencil order
+// (x -4..4, y -4..-1,1..4, z -4..-1,1..4) with fma, so results are bit-identical to it.
+#pragma once
+
+#include "kernels.h"
+
+namespace octhip {
+
+constexpr int ZT = 8;                // tile columns per side
+constexpr int ZR = 4;                // stencil reach
+constexpr int ZW = ZT + 2 * ZR;      // window side (tile + halo)
+constexpr int ZS = 192;              // sites of the cross-shaped plane window: 64 tile + 128 halo
+constexpr int ZPADZ = 8;             // march-cube padding below/above the inner points in z
+
+struct ZArgs {
+  ApplyArgs a;
+  const int* __restrict__ mcube;     // march cube: site -> point index or -1
+  int mx, my;                        // x, y extent of the march cube
+  const int4* __restrict__ items;    // {tile x, tile y, first plane, end plane} (cube z)
+  const int2* __restrict__ sched;    // blockIdx -> {item, lane chunk}
+};
+
+template <int LPT, int LC, int EPI, bool NL>
+__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
+  constexpr int PAIRS = LC / LPT;    // threads per column
+  constexpr int NT = 64 * PAIRS;
+  extern __shared__ double2 zsh2[];  // 16-byte aligned: row chunks move as ds_*_b128
+  double* const zsh = reinterpret_cast<double*>(zsh2);
+  double* const lds0 = zsh;                              // plane buffer 0
+  double* const lds1 = zsh + ZS * LC;                    // plane buffer 1
+  int* const tidx = reinterpret_cast<int*>(zsh + 2 * ZS * LC);  // item index table
+  const ApplyArgs& a = za.a;
+  const int2 sc = za.sched[blockIdx.x];
+  if (sc.x < 0) return;              // padding entry of the schedule (whole workgroup)
+  const int4 it = za.items[sc.x];
+  const int kp = threadIdx.x % PAIRS;
+  const int col = threadIdx.x / PAIRS;
+  const int cx = col & 7, cy = col >> 3;
+  const int l = sc.y * LC + kp * LPT;          // first lane of this thread
+  const int x0 = it.x * ZT + ZR, y0 = it.y * ZT + ZR;
+  const long mxl = za.mx;
+  const long sxy = mxl * za.my;
+  const size_t L = (size_t)a.L;
+  const int* __restrict__ mc = za.mcube;
+  // two halo units per thread: h = col + 64*i, same lane pair as the thread
+  int hlds[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int h = col + 64 * i;
+    int hx, hy, lx, ly;
+    if (h < 32) {
+      hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); lx = h & 3; ly = 4 + (h >> 2);
+    } else if (h < 64) {
+      const int g = h - 32;
+      hx = x0 + 8 + (g & 3); hy = y0 + (g >> 2); lx = 12 + (g & 3); ly = 4 + (g >> 2);
+    } else if (h < 96) {
+      const int g = h - 64;
+      hx = x0 + (g & 7); hy = y0 - 4 + (g >> 3); lx = 4 + (g & 7); ly = g >> 3;
+    } else {
+      const int g = h - 96;
+      hx = x0 + (g & 7); hy = y0 + 8 + (g >> 3); lx = 4 + (g & 7); ly = 12 + (g >> 3);
+    }
+    (void)hx;
+    (void)hy;
+    (void)lx;
+    (void)ly;
+    hlds[i] = (64 + h) * LC + kp * LPT;
+  }
+  // Cross-shaped window in LDS, one LC-lane chunk per site slot:
+  //   slot c (0..63)          tile column c = cx + 8 cy
+  //   slot 64 + 4 r + k       left strip,  x = x0-4+k, y = y0+r   (k < 4, r < 8)
+  //   slot 96 + 4 r + k       right strip, x = x0+8+k, y = y0+r
+  //   slot 128 + 8 k + c      bottom strip, x = x0+c, y = y0-4+k
+  //   slot 160 + 8 k + c      top strip,    x = x0+c, y = y0+8+k
+  const int own_lds = col * LC + kp * LPT;
+  auto xnb = [&](int dx) {  // LDS offset of the x-neighbour dx of this thread's column
+    const int xx = cx + dx;
+    const int slot = xx < 0 ? 64 + 4 * cy + (xx + 4) : (xx >= 8 ? 96 + 4 * cy + (xx - 8) : xx + 8 * cy);
+    return slot * LC + kp * LPT;
+  };
+  auto ynb = [&](int dy) {  // LDS offset of the y-neighbour dy
+    const int yy = cy + dy;
+    const int slot = yy < 0 ? 128 + 8 * (yy + 4) + cx : (yy >= 8 ? 160 + 8 * (yy - 8) + cx : cx + 8 * yy);
+    return slot * LC + kp * LPT;
+  };
+
+  // value of point q (zero row for boundary/non-points when q >= lim): branch-free load
+  auto val = [&](int q) {
+    const double* src = ((unsigned)q < (unsigned)a.lim) ? a.fi + (size_t)q * L + l : a.zrow;
+    return ldv<LPT>(src);
+  };
+  auto lds_st = [&](int buf, int off, const Vec<LPT>& v) {
+    double* b = buf ? lds1 : lds0;
+    stv<LPT>(b + off, v);
+  };
+  auto lds_ld = [&](int buf, int off) {
+    const double* b = buf ? lds1 : lds0;
+    return ldv<LPT>(b + off);
+  };
+
+  const int zb = it.z, ze = it.w;
+  int* const tks = tidx + (ze - zb + 8) * 192;  // ket slots of the own sites
+  // ---- index table of the item in LDS: 192 sites per plane (64 own columns, then the
+  //      128 halo sites), planes zb-4 .. ze+3; halo entries exist for planes zb .. ze-1.
+  //      Index reads then cost LDS, not the in-order global-load counter.
+  {
+    const int total = (ze - zb + 8) * 192;
+    for (int base = 0; base < total; base += NT * 8) {
+      int tq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = base + threadIdx.x + u * NT;
+        tq[u] = -1;
+        if (e < total) {
+          const int r = e / 192, t = e - r * 192;
+          const int P = zb - 4 + r;
+          long xy;
+          if (t < 64) {
+            xy = (x0 + (t & 7)) + mxl * (y0 + (t >> 3));
+          } else {
+            const int h = t - 64;
+            int hx, hy;
+            if (h < 32) { hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); }
+            else if (h < 64) { hx = x0 + 8 + ((h - 32) & 3); hy = y0 + ((h - 32) >> 2); }
+            else if (h < 96) { hx = x0 + ((h - 64) & 7); hy = y0 - 4 + ((h - 64) >> 3); }
+            else { hx = x0 + ((h - 96) & 7); hy = y0 + 8 + ((h - 96) >> 3); }
+            xy = hx + mxl * hy;
+          }
+          if (t < 64 || (P >= zb && P < ze)) tq[u] = mc[xy + sxy * P];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = base + threadIdx.x + u * NT;
+        if (e < total) tidx[e] = tq[u];
+      }
+    }
+    if constexpr (NL) {
+      __syncthreads();
+      const int totk = (ze - zb + 8) * 64;
+      for (int base = 0; base < totk; base += NT * 8) {
+        int kq[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + threadIdx.x + u * NT;
+          kq[u] = -1;
+          if (e < totk) {
+            const int q = tidx[(e >> 6) * 192 + (e & 63)];
+            if ((unsigned)q < (unsigned)a.np) kq[u] = a.kslot[q];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + threadIdx.x + u * NT;
+          if (e < totk) tks[e] = kq[u];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  auto own_idx = [&](int P) { return tidx[(P - zb + 4) * 192 + col]; };
+  auto halo_idx = [&](int P, int i) { return tidx[(P - zb + 4) * 192 + 64 + col + 64 * i]; };
+
+  // Rings with static slots (the z loop is unrolled by RQ, so every slot index is a
+  // compile-time constant and values stay in the registers their loads wrote):
+  //   ov[(P - zb + 4) % RQ]  own value of plane P,  P in z-4 .. z+6 (loaded 6 planes ahead)
+  //   hv[(P - zb) % 4]       halo values of plane P, P in z .. z+2  (loaded 2 planes ahead)
+  //   vr/er/ar[(P - zb) % 4] output point data,      P in z .. z+3  (loaded 3 planes ahead)
+  constexpr int RQ = 12;
+  Vec<LPT> ov[RQ];
+  Vec<LPT> hv[4][2];
+  double vr[4] = {0.0, 0.0, 0.0, 0.0};
+  Vec<LPT> kr[4];
+  Vec<LPT> ar[4];
+  const bool has_acc = (EPI == EPI_TAYLOR) && a.acc != nullptr;
+  // per-point data of plane P's output point: potential, ket sum row, accumulator row
+  auto load_aux = [&](int slot, int P) {
+    const int q = own_idx(P);
+    if ((unsigned)q < (unsigned)a.np) {
+      if (a.v) vr[slot] = a.v[q];
+      if constexpr (NL) {
+        const int ks = tks[(P - zb + 4) * 64 + col];
+        const double* src = ks >= 0 ? a.ksum + (size_t)ks * L + l : a.zrow;
+        kr[slot] = ldv<LPT>(src);
+      }
+      if (has_acc) ar[slot] = ldv<LPT>(a.acc + (size_t)q * L + l);
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 10; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) hv[k][i] = (zb + k < ze) ? val(halo_idx(zb + k, i)) : ov[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (zb + k < ze) load_aux(k, zb + k);
+
+  for (int z0 = zb; z0 < ze; z0 += RQ) {
+#pragma unroll
+    for (int kk = 0; kk < RQ; ++kk) {
+      const int z = z0 + kk;
+      if (z >= ze) break;
+      const int buf = kk & 1;
+      // prefetch: output-point data of plane z+3, own column z+6, halo of plane z+2
+      if (z + 3 < ze) load_aux((kk + 3) & 3, z + 3);
+      if (z + 6 < ze + 4) ov[(kk + 10) % RQ] = val(own_idx(z + 6));
+      if (z + 2 < ze) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) hv[(kk + 2) & 3][i] = val(halo_idx(z + 2, i));
+      }
+      // stage plane z: tile centre and halo strips
+      lds_st(buf, own_lds, ov[(kk + 4) % RQ]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) lds_st(buf, hlds[i], hv[kk & 3][i]);
+      __syncthreads();
+      // output point (x0+cx, y0+cy, z)
+      const int q0 = own_idx(z);
+      if ((unsigned)q0 < (unsigned)a.np) {
+        Vec<LPT> s;
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) s.v[k] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 25; ++j) {
+          Vec<LPT> x;
+          if (j < 9) {
+            const int dx = j - 4;
+            x = (dx == 0) ? ov[(kk + 4) % RQ] : lds_ld(buf, xnb(dx));
+          } else if (j < 17) {
+            const int dy = (j < 13) ? j - 13 : j - 12;
+            x = lds_ld(buf, ynb(dy));
+          } else {
+            const int dz = (j < 21) ? j - 21 : j - 20;
+            x = ov[(kk + 4 + dz + RQ) % RQ];
+          }
+#pragma unroll
+          for (int k = 0; k < LPT; ++k) s.v[k] = fma(a.w[j], x.v[k], s.v[k]);
+        }
+        const Vec<LPT> x = ov[(kk + 4) % RQ];
+        if (a.v) {
+          const double vv = vr[kk & 3];
+#pragma unroll
+          for (int k = 0; k < LPT; ++k) s.v[k] = fma(vv, x.v[k], s.v[k]);
+        }
+        if constexpr (NL) {
+#pragma unroll
+          for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kr[kk & 3].v[k];
+        }
+        if constexpr (EPI == EPI_STORE) {
+          stv<LPT>(a.fo + (size_t)q0 * L + l, s);
+        } else {
+          if (a.yout) stv<LPT>(a.yout + (size_t)q0 * L + l, s);
+          if (has_acc) {
+            Vec<LPT> av = ar[kk & 3];
+            if constexpr (LPT == 2) {
+              if (a.has_c1) {
+                av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], x.v[1], false);
+                av.v[1] = av.v[1] + zmul_lane(a.c1r, a.c1i, x.v[1], x.v[0], true);
+              }
+              av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], s.v[1], false);
+              av.v[1] = av.v[1] + zmul_lane(a.c2r, a.c2i, s.v[1], s.v[0], true);
+            } else {
+              // the partner lane of a (re, im) pair is held by the neighbouring thread
+              const bool odd = (l & 1) != 0;
+              if (a.has_c1) av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
+              av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
+            }
+            stv<LPT>(a.acc + (size_t)q0 * L + l, av);
+          }
+        }
+      }
+    }
+  }
+}
+
+// dynamic LDS bytes of k_zmarch for a lane chunk and segment length
+inline size_t zmarch_shmem(int lc, int zseg) {
+  return sizeof(double) * 2 * ZS * lc + sizeof(int) * (size_t)(zseg + 8) * (192 + 64);
+}
+
+}  // namespace octhip

@@ -1,0 +1,58 @@
+// Instances and launcher of the z-march apply.
+#include "zmarch.h"
+
+namespace octhip {
+
+namespace {
+template <int LPT, int LC, bool NL>
+int launch_nl(const ZArgs& z, int epi, int grid, size_t shmem, hipStream_t s) {
+  constexpr int NT = 64 * LC / LPT;
+  if (epi == EPI_TAYLOR) {
+    if constexpr (LC >= 2) {
+      hipLaunchKernelGGL((k_zmarch<LPT, LC, EPI_TAYLOR, NL>), dim3(grid), dim3(NT), shmem, s, z);
+    } else {
+      return fail(OCTHIP_ERR_INVALID, "z-march Taylor epilogue needs complex lane pairs");
+    }
+  } else {
+    hipLaunchKernelGGL((k_zmarch<LPT, LC, EPI_STORE, NL>), dim3(grid), dim3(NT), shmem, s, z);
+  }
+  return OCTHIP_SUCCESS;
+}
+
+template <int LPT, int LC>
+int launch_lc(const ZArgs& z, int epi, int grid, int zseg, hipStream_t s) {
+  const size_t shmem = zmarch_shmem(LC, zseg);
+  ProfScope prof("k_apply", s);
+  const int rc = z.a.kslot ? launch_nl<LPT, LC, true>(z, epi, grid, shmem, s)
+                         : launch_nl<LPT, LC, false>(z, epi, grid, shmem, s);
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zmarch launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, hipStream_t s) {
+  if (lpt == 2) {
+    switch (lc) {
+      case 2: return launch_lc<2, 2>(z, epi, grid, zseg, s);
+      case 4: return launch_lc<2, 4>(z, epi, grid, zseg, s);
+      case 8: return launch_lc<2, 8>(z, epi, grid, zseg, s);
+      case 16: return launch_lc<2, 16>(z, epi, grid, zseg, s);
+      case 32: return launch_lc<2, 32>(z, epi, grid, zseg, s);
+      default: break;
+    }
+  } else {
+    switch (lc) {
+      case 1: return launch_lc<1, 1>(z, epi, grid, zseg, s);
+      case 2: return launch_lc<1, 2>(z, epi, grid, zseg, s);
+      case 4: return launch_lc<1, 4>(z, epi, grid, zseg, s);
+      case 8: return launch_lc<1, 8>(z, epi, grid, zseg, s);
+      case 16: return launch_lc<1, 16>(z, epi, grid, zseg, s);
+      default: break;
+    }
+  }
+  return fail(OCTHIP_ERR_INVALID, "no z-march instance for lane chunk %d, %d lanes per thread", lc, lpt);
+}
+
+}  // namespace octhip

@@ -16,6 +16,7 @@ ERR_INVALID, ERR_HIP, ERR_ALLOC, ERR_UNSUPPORTED, ERR_STATE = -1, -2, -3, -4, -5
 OP_INNER, OP_OUTER, OP_ALL = 1, 2, 3
 TERM_KINETIC, TERM_LOCAL_POTENTIAL, TERM_NON_LOCAL_POTENTIAL = 1, 2, 4
 TERM_ALL = 0x7FFFFFFF
+TUNE = dict(kernel=1, xcd_chunk=2, gather_tile=3, zseg=4, lane_chunk=5, lanes_per_thread=6, bra_mode=16)
 
 # (name, restype-is-int, argtypes) for every symbol of include/octhip.h
 _P = C.c_void_p
@@ -49,8 +50,9 @@ SIGNATURES = {
     "octhip_profiling_start": [],
     "octhip_profiling_stop": [],
     "octhip_profiling_get": [C.c_char_p, _DP, _IP],
-    "octhip_nl_operator_set_schedule": [_P, _I],
-    "octhip_hamiltonian_set_bra_mode": [_P, _I],
+    "octhip_nl_operator_set_tuning": [_P, _I, _I],
+    "octhip_nl_operator_get_tuning": [_P, _I, _IP],
+    "octhip_hamiltonian_set_tuning": [_P, _I, _I],
 }
 
 

@@ -98,8 +98,16 @@ class NLOperatorDevice:
              C.byref(h))
         self.handle = h
 
-    def set_schedule(self, xcd_chunk: int) -> None:
-        call("octhip_nl_operator_set_schedule", self.handle, int(xcd_chunk))
+    def tune(self, **knobs) -> None:
+        """Set tuning knobs by name: kernel (0 auto, 1 gather, 2 z-march), xcd_chunk,
+        gather_tile, zseg, lane_chunk, lanes_per_thread."""
+        for k, v in knobs.items():
+            call("octhip_nl_operator_set_tuning", self.handle, _lib.TUNE[k], int(v))
+
+    def tuning(self, key: str) -> int:
+        v = C.c_int()
+        call("octhip_nl_operator_get_tuning", self.handle, _lib.TUNE[key], C.byref(v))
+        return v.value
 
     @property
     def uses_lattice(self) -> bool:
@@ -149,9 +157,10 @@ class HamiltonianDevice:
         kin = NLOperatorDevice(system.op, star_points(4) if lattice else None)
         return cls(kin, system.v, system.proj, mass)
 
-    def set_bra_mode(self, mode: int) -> None:
-        """0 = plain FMA reduction, 1 = FP64 MFMA (default)."""
-        call("octhip_hamiltonian_set_bra_mode", self.handle, int(mode))
+    def tune(self, **knobs) -> None:
+        """bra_mode (0 plain, 1 FP64 MFMA) plus the kinetic operator's knobs."""
+        for k, v in knobs.items():
+            call("octhip_hamiltonian_set_tuning", self.handle, _lib.TUNE[k], int(v))
 
     def set_potential(self, v: np.ndarray) -> None:
         v = np.ascontiguousarray(v, dtype=np.float64)

@@ -31,37 +31,46 @@ def run_operate(gpu, op_dev, fin, factor):
     return fo.ff.cpu().numpy()
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 8, 64, 128])
-@pytest.mark.parametrize("lattice", [True, False])
-def test_stencil_bit_exact_sphere(gpu, lanes, lattice):
-    print(f"[stencil sphere lanes={lanes} lattice={lattice}]", flush=True)
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64, 128])
+@pytest.mark.parametrize("kernel", ["ri", "gather", "zmarch"])
+def test_stencil_bit_exact_sphere(gpu, lanes, kernel):
+    print(f"[stencil sphere lanes={lanes} kernel={kernel}]", flush=True)
     s = small_sphere(radius=3.0, spacing=0.25)
-    op = api.NLOperatorDevice(s.op, star_points(4) if lattice else None)
-    assert op.uses_lattice == lattice
+    op = api.NLOperatorDevice(s.op, None if kernel == "ri" else star_points(4))
+    assert op.uses_lattice == (kernel != "ri")
+    if kernel != "ri":
+        if kernel == "gather":
+            op.tune(kernel=1, gather_tile=3 if lanes == 8 else 4)
+        else:
+            op.tune(kernel=2, zseg=5 if lanes == 8 else 16, lanes_per_thread=1 if lanes in (4, 64) else 2)
     f = random_packed(s.np_part, lanes, seed=lanes)
     got = run_operate(gpu, op, f, -0.5)
     ref = O.operate_ri_vec(s.op, f, -0.5)
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("chunk", [1, 4, 64])
-def test_stencil_bit_exact_cube_workload(gpu, chunk):
-    """The 64^3 cube workload (8 Gaussian sums, pack 8) at full size; schedules only
-    reorder tiles, so every schedule is bit-identical."""
-    print(f"[stencil cube64 chunk={chunk}]", flush=True)
+@pytest.mark.parametrize("knobs", [dict(kernel=1, xcd_chunk=1), dict(kernel=1, xcd_chunk=64, gather_tile=2),
+                                   dict(kernel=2), dict(kernel=2, zseg=7, lane_chunk=4),
+                                   dict(kernel=2, lanes_per_thread=1)])
+def test_stencil_bit_exact_cube_workload(gpu, knobs):
+    """The 64^3 cube workload (8 Gaussian sums, pack 8) at full size; kernels and
+    schedules only reorder work, so every choice is bit-identical."""
+    print(f"[stencil cube64 {knobs}]", flush=True)
     s = workloads.laplacian_system()
     f = workloads.gaussian_functions(s, 8)
     packed = workloads.pack(f, s.np_part, 8)
     op = api.NLOperatorDevice(s.op, star_points(4))
-    op.set_schedule(chunk)
+    op.tune(**knobs)
     got = run_operate(gpu, op, packed, 1.0)
     ref = O.operate_ri_vec(s.op, packed, 1.0)
     assert np.array_equal(got, ref)
 
 
-def run_h(gpu, s, z, terms=api.TERM_ALL, set_bc=True, lattice=True, bra=1, pack_size=None):
+def run_h(gpu, s, z, terms=api.TERM_ALL, set_bc=True, lattice=True, bra=1, pack_size=None, kernel=0):
     hm = api.HamiltonianDevice.from_system(s, lattice=lattice)
-    hm.set_bra_mode(bra)
+    hm.tune(bra_mode=bra)
+    if lattice:
+        hm.tune(kernel=kernel)
     psib = api.Batch.from_states(z, s.np_part, pack_size)
     hpsib = api.Batch(psib.nst, s.np, psib.is_complex, psib.pack_size)
     api.hamiltonian_elec_apply_batch(hm, psib, hpsib, terms, set_bc)
@@ -80,11 +89,12 @@ def states(s, nst, seed=0):
 
 @pytest.mark.parametrize("terms", [api.TERM_KINETIC, api.TERM_LOCAL_POTENTIAL,
                                    api.TERM_KINETIC | api.TERM_LOCAL_POTENTIAL])
-def test_hamiltonian_local_terms_bit_exact(gpu, terms):
-    print(f"[hamiltonian terms={terms}]", flush=True)
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_hamiltonian_local_terms_bit_exact(gpu, terms, kernel):
+    print(f"[hamiltonian terms={terms} kernel={kernel}]", flush=True)
     s = small_sphere(radius=3.0, spacing=0.25, natoms=4)
     z = states(s, 16)
-    assert np.array_equal(run_h(gpu, s, z, terms), oracle_h(s, z, terms))
+    assert np.array_equal(run_h(gpu, s, z, terms, kernel=kernel), oracle_h(s, z, terms))
 
 
 @pytest.mark.parametrize("bra", [0, 1])
@@ -132,23 +142,28 @@ def test_hamiltonian_no_projectors(gpu):
 
 
 def test_hamiltonian_workload_batch(gpu):
-    """One 32-state batch of the R=12, h=0.25 sphere workload (462,781 points)."""
+    """One 32-state batch of the R=12, h=0.25 sphere workload (462,781 points), through
+    both apply kernels; the two kernels agree bit for bit."""
     print("[hamiltonian sphere R=12 batch]", flush=True)
     s = workloads.hamiltonian_system()
     z = workloads.gaussian_envelope_states(s, 32)
-    got, ref = run_h(gpu, s, z), oracle_h(s, z)
-    assert max_rel_err(got, ref) <= 1e-12
+    ref = oracle_h(s, z)
+    got_z, got_g = run_h(gpu, s, z, kernel=2), run_h(gpu, s, z, kernel=1)
+    assert max_rel_err(got_z, ref) <= 1e-12
+    assert np.array_equal(got_z, got_g)
 
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("imag_time", [False, True])
-def test_taylor_step(gpu, order, imag_time):
-    print(f"[taylor order={order} imag={imag_time}]", flush=True)
+@pytest.mark.parametrize("kernel,lpt", [(1, 0), (2, 2), (2, 1)])
+def test_taylor_step(gpu, order, imag_time, kernel, lpt):
+    print(f"[taylor order={order} imag={imag_time} kernel={kernel} lpt={lpt}]", flush=True)
     s = small_sphere(radius=3.0, spacing=0.3, natoms=3, rc=1.2)
     z = workloads.orthonormalise(states(s, 8), s.mesh.vol_pp)
     psi = workloads.pack(z, s.np_part)
     O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, 0.01, order, imag_time)
     hm = api.HamiltonianDevice.from_system(s)
+    hm.tune(kernel=kernel, lanes_per_thread=lpt)
     psib = api.Batch.from_states(z, s.np_part)
     api.Exponential(order).apply_batch(hm, psib, 0.01, imag_time)
     gpu.cuda.synchronize()
