@@ -107,7 +107,7 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     _lib.profiling_stop()
     ms_step = (t1 - t0) * 1e3 / args.steps
     kern = {}
-    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_zero_rows"):
+    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_zero_rows"):
         tot, n = _lib.profiling_get(k)
         kern[k] = {"avg_us": tot * 1e3 / n if n else None, "launches": n}
     bytes_batch = hamiltonian_bytes(s, pack)

@@ -27,7 +27,7 @@ int ensure_work(octhip_hamiltonian_s* h, int ldf) {
   if (h->nmat > 0) {
     rc = dalloc((void**)&h->d_partial, sizeof(double) * h->nchunk * h->max_nprojs * L);
     if (!rc) rc = dalloc((void**)&h->d_proj, sizeof(double) * h->full_projection_size * L);
-    if (!rc) rc = dalloc((void**)&h->d_ksum, sizeof(double) * h->nslot * L);
+    if (!rc) rc = dalloc((void**)&h->d_ksum, sizeof(double) * ((size_t)h->total_points + h->nmulti) * L);
   }
   if (rc) return rc;
   h->work_ldf = ldf;
@@ -77,18 +77,19 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s) {
   rc = launch_bra_reduce(r, s);
   if (rc) return rc;
   KetArgs k;
-  k.slot_pt = h->d_slot_pt;
-  k.pos = h->d_pos;
-  k.ent = h->d_ent;
-  k.pt = h->d_pt;
+  k.chunks = h->d_chunks;
+  k.mat_info = h->d_mat_info;
   k.mat_pbase = h->d_mat_pbase;
-  k.mat_npj = h->d_mat_npj;
+  k.pt = h->d_pt;
   k.pjs = h->pjs;
   k.proj = h->d_proj;
-  k.ksum = h->d_ksum;
-  k.nslot = h->nslot;
+  k.kbuf = h->d_ksum;
   k.L = L;
-  return launch_ket(k, s);
+  k.multi_pos = h->d_multi_pos;
+  k.multi_g = h->d_multi_g;
+  k.nmulti = h->nmulti;
+  k.total_points = h->total_points;
+  return launch_ket(k, h->nchunk, h->use_mfma, s);
 }
 
 // v_eff(p) = v(p) + (f0 x + f1 y + f2 z): the dipole term of laser_potential
@@ -216,15 +217,26 @@ extern "C" int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double ma
       }
     }
     h->nchunk = (int)chunks.size();
-    std::vector<int> kslot(np, -1), slot_pt;
-    for (int p = 0; p < np; ++p)
-      if (pos[p + 1] > pos[p]) {
-        kslot[p] = (int)slot_pt.size();
-        slot_pt.push_back(p);
+    // ket row per mesh point: its sphere point if it lies in one sphere, else a summed row
+    std::vector<int> kslot(np, -1), multi_pos(1, 0), multi_g;
+    int nslot = 0;
+    for (int p = 0; p < np; ++p) {
+      const int ne = pos[p + 1] - pos[p];
+      if (ne == 0) continue;
+      ++nslot;
+      if (ne == 1) {
+        kslot[p] = ent[pos[p]].x;
+      } else {
+        kslot[p] = (int)total_points + (int)multi_pos.size() - 1;
+        for (int e = pos[p]; e < pos[p + 1]; ++e) multi_g.push_back(ent[e].x);
+        multi_pos.push_back((int)multi_g.size());
       }
-    h->nslot = (int)slot_pt.size();
+    }
+    h->nslot = nslot;
+    h->nmulti = (int)multi_pos.size() - 1;
     rc = upload(&h->d_kslot, kslot.data(), kslot.size(), s);
-    if (!rc) rc = upload(&h->d_slot_pt, slot_pt.data(), slot_pt.size(), s);
+    if (!rc) rc = upload(&h->d_multi_pos, multi_pos.data(), multi_pos.size(), s);
+    if (!rc && !multi_g.empty()) rc = upload(&h->d_multi_g, multi_g.data(), multi_g.size(), s);
     if (!rc) rc = upload(&h->d_matrices, matrices, matrix_size, s);
     if (!rc) rc = upload(&h->d_maps, maps, total_points, s);
     if (!rc) rc = upload(&h->d_scals, scals, nproj, s);
@@ -252,7 +264,8 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_mat_info);
   dfree(h->d_proj_mat);
   dfree(h->d_kslot);
-  dfree(h->d_slot_pt);
+  dfree(h->d_multi_pos);
+  dfree(h->d_multi_g);
   dfree(h->d_ksum);
   dfree(h->d_v);
   dfree(h->d_xyz);

@@ -252,24 +252,30 @@ struct BraReduceArgs {
   int L, pjmax, nproj;
 };
 
-// ksum[slot, l] = sum over the point's (sphere point, matrix) entries, in matrix order, of
-// sum_j P[sphere point, j] proj[matrix base + j, l]  (the ket of X(hamiltonian_elec_base_
-// nlocal_finish), src/hamiltonian/hamiltonian_elec_base_inc.F90:914-943, per mesh point)
+// Ket of X(hamiltonian_elec_base_nlocal_finish) (src/hamiltonian/hamiltonian_elec_base_inc.F90:
+// 914-943) in two passes:
+//  k_ket:     kbuf[g, l] = sum_j P[g, j] proj[pbase(g) + j, l] for every global sphere point g
+//             (matrix by matrix: a (npoints x nprojs) x (nprojs x L) product, FP64 MFMA);
+//  k_ket_sum: kbuf[total_points + m, l] = sum of the rows of the m-th mesh point that lies in
+//             several spheres, in matrix order.
+// A mesh point p then adds row kref[p] of kbuf (its single sphere row or its sum), or nothing.
 struct KetArgs {
-  const int* __restrict__ slot_pt;   // slot -> mesh point
-  const int* __restrict__ pos;       // CSR of entries per mesh point
-  const int2* __restrict__ ent;      // (global sphere point, matrix)
-  const double* __restrict__ pt;     // transposed projector rows, stride pjs
+  const int4* __restrict__ chunks;    // bra chunks {matrix, q0, q1, -}
+  const int4* __restrict__ mat_info;  // {npoints, nprojs, matrix offset, map offset}
   const int* __restrict__ mat_pbase;
-  const int* __restrict__ mat_npj;
+  const double* __restrict__ pt;      // transposed projector rows, stride pjs
   int pjs;
   const double* __restrict__ proj;
-  double* __restrict__ ksum;
-  int nslot, L;
+  double* __restrict__ kbuf;
+  int L;
+  // multi-sphere sums
+  const int* __restrict__ multi_pos;  // nmulti+1 CSR into multi_g
+  const int* __restrict__ multi_g;    // global sphere points, matrix order
+  int nmulti, total_points;
 };
 
 int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s);
-int launch_ket(const KetArgs& k, hipStream_t s);
+int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s);
 int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s);
 
 // sphere points per bra chunk

@@ -118,6 +118,7 @@ struct BraMfmaArgs {
 };
 
 __global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
+  constexpr int KB = 16;                    // MFMA steps (4 sphere points each) per load block
   __shared__ double red[4 * 16 * 16];
   const BraArgs& b = m.b;
   const int c = blockIdx.x;
@@ -133,19 +134,29 @@ __global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
   const int my_sl = nlt >= 4 ? 0 : wave / nlt;
   const int t0 = nlt >= 4 ? wave : wave % nlt;
   const int tstep = nlt >= 4 ? 4 : nlt;
+  // operand loads of one MFMA step: lane (kk, ii) supplies A[ii][kk] and B[kk][ii]
+  auto load_ab = [&](int q, int jb, int col, double& av, double& bv) {
+    const int qk = q + kk;
+    av = 0.0;
+    bv = 0.0;
+    if (qk < ch.z) {
+      if (jb + ii < npj) av = m.pt[(size_t)(mapoff + qk) * m.pjs + jb + ii];
+      const int row = b.maps[mapoff + qk] - 1;
+      bv = b.psi[(size_t)row * b.L + col];
+    }
+  };
   for (int jb = 0; jb < npj; jb += 16) {
     for (int lt = t0; lt < nlt; lt += tstep) {
       d4 acc = {0.0, 0.0, 0.0, 0.0};
       const int col = lt * 16 + ii;
-      for (int q = ch.y + 4 * my_sl; q < ch.z; q += 4 * nsl) {
-        const int qk = q + kk;
-        double av = 0.0, bv = 0.0;
-        if (qk < ch.z) {
-          if (jb + ii < npj) av = m.pt[(size_t)(mapoff + qk) * m.pjs + jb + ii];
-          const int row = b.maps[mapoff + qk] - 1;
-          bv = b.psi[(size_t)row * b.L + col];
-        }
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      const int qstep = 4 * nsl;
+      // blocks of KB steps; each block's loads are issued together before its MFMAs
+      for (int q0 = ch.y + 4 * my_sl; q0 < ch.z; q0 += KB * qstep) {
+        double av[KB], bv[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) load_ab(q0 + k * qstep, jb, col, av[k], bv[k]);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], bv[k], acc, 0, 0, 0);
       }
       if (nsl == 1) {
 #pragma unroll
@@ -184,8 +195,15 @@ __global__ void k_bra_reduce(const BraReduceArgs r) {
     const int a = r.proj_mat[i];
     const int j = i - r.mat_pbase[a];
     double s = 0.0;
-    for (int c = r.mat_chunk0[a]; c < r.mat_chunk0[a + 1]; ++c)
-      s += r.partial[((size_t)c * r.pjmax + j) * r.L + l];
+    const int c0 = r.mat_chunk0[a], c1 = r.mat_chunk0[a + 1];
+    for (int c = c0; c < c1; c += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (c + u < c1) ? r.partial[((size_t)(c + u) * r.pjmax + j) * r.L + l] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (c + u < c1) s += v[u];
+    }
     r.proj[t] = s * r.scals[i];
   }
 }
@@ -214,35 +232,84 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
   return OCTHIP_SUCCESS;
 }
 
-__global__ void k_ket(const KetArgs k) {
-  const size_t total = (size_t)k.nslot * k.L;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const int sl = (int)(t / k.L);
-    const int l = (int)(t - (size_t)sl * k.L);
-    const int p = k.slot_pt[sl];
-    double s = 0.0;
-    for (int e = k.pos[p]; e < k.pos[p + 1]; ++e) {
-      const int2 en = k.ent[e];
-      const int pb = k.mat_pbase[en.y];
-      const int nj = k.mat_npj[en.y];
-      const double* prow = k.pt + (size_t)en.x * k.pjs;
-      double t2 = 0.0;
-      for (int jj = 0; jj < nj; ++jj) t2 = fma(prow[jj], k.proj[(size_t)(pb + jj) * k.L + l], t2);
-      s = s + t2;
+// One workgroup per bra chunk (<= BRA_CHUNK sphere points of one matrix).  MFMA form:
+// D(16 points x 16 lanes) = A(16 points x 4 projectors) B(4 projectors x 16 lanes); lane
+// (kk, ii) supplies A[ii][kk] = P[q0+ii, jb+kk] and B[kk][ii] = proj[pbase+jb+kk, col];
+// accumulator element r of lane (kk, ii) is D[4r+kk][ii].
+__global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) {
+  const int4 ch = k.chunks[blockIdx.x];
+  const int4 mi = k.mat_info[ch.x];
+  const int npj = mi.y, mapoff = mi.w;
+  const int pb = k.mat_pbase[ch.x];
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int kk = ln >> 4, ii = ln & 15;
+  const int nlt = k.L >> 4;
+  const int npt = ch.z - ch.y;
+  const int ntile = ((npt + 15) >> 4) * nlt;
+  for (int tl = wave; tl < ntile; tl += 4) {
+    const int pt0 = ch.y + (tl / nlt) * 16;
+    const int col = (tl % nlt) * 16 + ii;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int jb = 0; jb < npj; jb += 4) {
+      const int q = pt0 + ii;
+      const double av = (q < ch.z && jb + kk < npj) ? k.pt[(size_t)(mapoff + q) * k.pjs + jb + kk] : 0.0;
+      const double bv = (jb + kk < npj) ? k.proj[(size_t)(pb + jb + kk) * k.L + col] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-    k.ksum[t] = s;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = pt0 + 4 * r + kk;
+      if (q < ch.z) k.kbuf[(size_t)(mapoff + q) * k.L + col] = acc[r];
+    }
   }
 }
 
-int launch_ket(const KetArgs& k, hipStream_t s) {
-  const size_t total = (size_t)k.nslot * k.L;
+// plain form, any row length: one thread per (sphere point, lane)
+__global__ __launch_bounds__(BLOCK) void k_ket_plain(const KetArgs k) {
+  const int4 ch = k.chunks[blockIdx.x];
+  const int4 mi = k.mat_info[ch.x];
+  const int npj = mi.y, mapoff = mi.w;
+  const int pb = k.mat_pbase[ch.x];
+  const int total = (ch.z - ch.y) * k.L;
+  for (int t = threadIdx.x; t < total; t += BLOCK) {
+    const int q = ch.y + t / k.L, l = t % k.L;
+    const double* prow = k.pt + (size_t)(mapoff + q) * k.pjs;
+    double acc = 0.0;
+    for (int jj = 0; jj < npj; ++jj) acc = fma(prow[jj], k.proj[(size_t)(pb + jj) * k.L + l], acc);
+    k.kbuf[(size_t)(mapoff + q) * k.L + l] = acc;
+  }
+}
+
+__global__ void k_ket_sum(const KetArgs k) {
+  const size_t total = (size_t)k.nmulti * k.L;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(t / k.L);
+    const int l = (int)(t - (size_t)m * k.L);
+    double s = 0.0;
+    for (int e = k.multi_pos[m]; e < k.multi_pos[m + 1]; ++e) s = s + k.kbuf[(size_t)k.multi_g[e] * k.L + l];
+    k.kbuf[((size_t)k.total_points + m) * k.L + l] = s;
+  }
+}
+
+int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
+  if (nchunk <= 0) return OCTHIP_SUCCESS;
+  {
+    ProfScope prof("k_ket", s);
+    if (mfma && k.L >= 16)
+      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk), dim3(BLOCK), 0, s, k);
+    else
+      hipLaunchKernelGGL(k_ket_plain, dim3(nchunk), dim3(BLOCK), 0, s, k);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_ket launch: %s", hipGetErrorString(e));
+  }
+  const size_t total = (size_t)k.nmulti * k.L;
   if (total == 0) return OCTHIP_SUCCESS;
-  int grid = (int)std::min<size_t>((total + BLOCK - 1) / BLOCK, 16384);
-  ProfScope prof("k_ket", s);
-  hipLaunchKernelGGL(k_ket, dim3(grid), dim3(BLOCK), 0, s, k);
+  const int grid = (int)std::min<size_t>((total + BLOCK - 1) / BLOCK, 8192);
+  ProfScope prof("k_ket_sum", s);
+  hipLaunchKernelGGL(k_ket_sum, dim3(grid), dim3(BLOCK), 0, s, k);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_ket launch: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_ket_sum launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
 

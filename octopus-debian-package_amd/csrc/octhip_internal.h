@@ -143,9 +143,11 @@ struct octhip_hamiltonian_s {
   int4* d_mat_info = nullptr;   // nmat: {npoints, nprojs, matrix offset, map offset}
   int* d_proj_mat = nullptr;    // projection index -> matrix
   int nslot = 0;                // mesh points inside at least one sphere
-  int* d_kslot = nullptr;       // np: slot of each mesh point or -1
-  int* d_slot_pt = nullptr;     // nslot: mesh point of each slot
-  double* d_ksum = nullptr;     // nslot * L ket sums
+  int nmulti = 0;               // mesh points inside several spheres
+  int* d_kslot = nullptr;       // np: row of the ket buffer for each mesh point, or -1
+  int* d_multi_pos = nullptr;   // nmulti+1
+  int* d_multi_g = nullptr;     // sphere points of the multi-sphere mesh points
+  double* d_ksum = nullptr;     // (total_points + nmulti) * L ket rows
   // work buffers sized for the largest ldf seen
   int work_ldf = -1;
   double* d_partial = nullptr;  // nchunk * max_nprojs * L
