@@ -55,6 +55,7 @@ extern "C" {
 
 typedef struct octhip_nl_operator_s* octhip_nl_operator_t;
 typedef struct octhip_hamiltonian_s* octhip_hamiltonian_t;
+typedef struct octhip_taylor_graph_s* octhip_taylor_graph_t;
 
 /* ---------------------------------------------------------------- lifecycle --
  * Replaces accel_init / accel_end (src/basic/accel.F90:264-700, 1105-1175). */
@@ -156,6 +157,22 @@ int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t hm, double* psi, double*
  * are owned by the handle and reused across calls. */
 int octhip_exponential_taylor_batch(octhip_hamiltonian_t hm, double* psi, int ldf,
                                     double deltat, int order, int imag_time, void* stream);
+
+/* One Taylor time step over nbatch batches (psi[b], device, same ldf) captured once into
+ * a HIP graph and replayed with a single launch: the loop over batches of
+ * propagation_ops_elec_exp_apply (src/td/propagation_ops_elec.F90:232-249), each
+ * te%apply_batch being octhip_exponential_taylor_batch, without per-kernel launch cost.  The graph holds the batch pointers,
+ * deltat, order and the handle's potential buffers as they are at capture;
+ * set_potential and set_dipole_field update those buffers in place, so a field that
+ * changes between launches is seen when one was set before capture.  The graph shares
+ * the handle's work buffers: launches that overlap with other work on the same handle
+ * race.  stream == NULL captures on a private stream; the graph may be launched on any
+ * stream. */
+int octhip_taylor_graph_create(octhip_hamiltonian_t hm, double* const* psi, int nbatch, int ldf,
+                               double deltat, int order, int imag_time, void* stream,
+                               octhip_taylor_graph_t* graph);
+int octhip_taylor_graph_launch(octhip_taylor_graph_t graph, void* stream);
+int octhip_taylor_graph_destroy(octhip_taylor_graph_t graph);
 
 /* --------------------------------------------------------------- profiling --
  * Per-kernel device timing with HIP events recorded on the launch stream, the device-side

@@ -10,6 +10,7 @@ namespace octhip {
 void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a);
 int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s);
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s);
+int prepare_operator(octhip_nl_operator_s* op, int L);
 }  // namespace octhip
 
 namespace {
@@ -317,8 +318,8 @@ extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const
   OCTHIP_REQUIRE(h && field, "null argument");
   hipStream_t s = (hipStream_t)stream;
   if (positions) {
+    // d_veff keeps its address (captured graphs read it); only the positions are replaced
     dfree(h->d_xyz);
-  dfree(h->d_veff);
     h->d_xyz = nullptr;
     int rc = upload(&h->d_xyz, positions, 3 * (size_t)h->np, s);
     if (rc) return rc;
@@ -359,18 +360,103 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   return launch_operator(h->kin, a, EPI_STORE, s);
 }
 
+namespace {
+int prepare_taylor(octhip_hamiltonian_s* h, int ldf) {
+  int rc = ensure_work(h, ldf);
+  if (!rc) rc = ensure_taylor(h, ldf);
+  if (!rc) rc = prepare_operator(h->kin, 1 << ldf);
+  return rc;
+}
+
+int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                hipStream_t s);
+}  // namespace
+
 extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
                                                int order, int imag_time, void* stream) {
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h && psi, "null argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  int rc = prepare_taylor(h, ldf);
+  if (rc) return rc;
+  return taylor_impl(h, psi, ldf, deltat, order, imag_time, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- graphs --
+struct octhip_taylor_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf,
+                                          double deltat, int order, int imag_time, void* stream,
+                                          octhip_taylor_graph_t* out) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && psi && out && nbatch > 0, "bad argument");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  *out = nullptr;
+  int rc = prepare_taylor(h, ldf);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  hipStream_t cs = s;
+  bool own = false;
+  if (!cs) {  // the legacy default stream cannot be captured
+    OCTHIP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    own = true;
+  }
+  OCTHIP_HIP(hipStreamSynchronize(cs));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    if (own) (void)hipStreamDestroy(cs);
+    return fail(OCTHIP_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+  }
+  set_capturing(true);
+  for (int b = 0; b < nbatch && !rc; ++b) rc = taylor_impl(h, psi[b], ldf, deltat, order, imag_time, cs);
+  set_capturing(false);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(cs, &graph);
+  if (own) (void)hipStreamDestroy(cs);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  auto* g = new octhip_taylor_graph_s();
+  g->graph = graph;
+  e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    delete g;
+    return fail(OCTHIP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_taylor_graph_launch(octhip_taylor_graph_t g, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(g != nullptr, "null graph");
+  ProfScope prof("taylor_graph", (hipStream_t)stream);
+  OCTHIP_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_taylor_graph_destroy(octhip_taylor_graph_t g) {
+  if (!g) return OCTHIP_SUCCESS;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return OCTHIP_SUCCESS;
+}
+
+namespace {
+int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                hipStream_t s) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
-  int rc = ensure_work(h, ldf);
-  if (!rc) rc = ensure_taylor(h, ldf);
-  if (!rc) rc = zero_boundary(psi, op->np, op->np_part, L, s);
+  int rc = zero_boundary(psi, op->np, op->np_part, L, s);
   if (rc) return rc;
   // zfact *= (-i * deltat) / k  (exponential.F90:720-724), deltat -> i*deltat in imaginary time
   const std::complex<double> dt_c = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
@@ -420,3 +506,4 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
   }
   return OCTHIP_SUCCESS;
 }
+}  // namespace

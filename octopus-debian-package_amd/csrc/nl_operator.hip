@@ -214,6 +214,20 @@ int zmarch_sched(octhip_nl_operator_s* op, int nchunk, const int2** out, int* le
 
 int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, hipStream_t s);
 
+// Build every lazily created device structure an apply with rows of L doubles needs, so
+// that later applies allocate nothing (required before a graph capture).
+int prepare_operator(octhip_nl_operator_s* op, int L) {
+  const bool zm = op->zmarch_ok && op->kernel != 1 && op->n == 25;
+  if (!zm) return OCTHIP_SUCCESS;
+  int rc = zmarch_items(op);
+  if (rc) return rc;
+  int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
+  if (lc > L) lc = L;
+  const int2* sched = nullptr;
+  int len = 0;
+  return zmarch_sched(op, L / lc, &sched, &len);
+}
+
 // Apply through the best kernel for this operator: z-march for the order-4 star on a
 // lattice, the tiled gather otherwise.
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s) {

@@ -60,6 +60,9 @@ int upload(T** dst, const T* src, size_t count, hipStream_t s) {
 int dalloc(void** p, size_t bytes);
 void dfree(void* p);
 
+// While a graph is being captured no profiling events are recorded.
+void set_capturing(bool on);
+
 // Device timing of one kernel launch when profiling is on (events on the launch stream).
 class ProfScope {
  public:

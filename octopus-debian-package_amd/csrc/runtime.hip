@@ -49,11 +49,14 @@ struct ProfRec {
   hipEvent_t start, stop;
 };
 bool g_prof = false;
+bool g_capturing = false;
 std::vector<ProfRec> g_recs;
 }  // namespace
 
+void set_capturing(bool on) { g_capturing = on; }
+
 ProfScope::ProfScope(const char* region, hipStream_t s) : s_(s) {
-  if (!g_prof) return;
+  if (!g_prof || g_capturing) return;
   ProfRec r{region, nullptr, nullptr};
   if (hipEventCreate(&r.start) != hipSuccess || hipEventCreate(&r.stop) != hipSuccess) return;
   if (hipEventRecord(r.start, s) != hipSuccess) return;

@@ -47,6 +47,9 @@ SIGNATURES = {
     "octhip_hamiltonian_set_dipole_field": [_P, _DP, _DP, _P],
     "octhip_hamiltonian_apply_batch": [_P, _P, _P, _I, _I, _I, _I, _P],
     "octhip_exponential_taylor_batch": [_P, _P, _I, _D, _I, _I, _P],
+    "octhip_taylor_graph_create": [_P, _P, _I, _I, _D, _I, _I, _P, _P],
+    "octhip_taylor_graph_launch": [_P, _P],
+    "octhip_taylor_graph_destroy": [_P],
     "octhip_profiling_start": [],
     "octhip_profiling_stop": [],
     "octhip_profiling_get": [C.c_char_p, _DP, _IP],

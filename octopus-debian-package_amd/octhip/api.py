@@ -117,7 +117,10 @@ class NLOperatorDevice:
 
     def __del__(self):
         if getattr(self, "handle", None):
-            _lib.lib().octhip_nl_operator_destroy(self.handle)
+            try:
+                _lib.lib().octhip_nl_operator_destroy(self.handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
             self.handle = None
 
 
@@ -174,7 +177,10 @@ class HamiltonianDevice:
 
     def __del__(self):
         if getattr(self, "handle", None):
-            _lib.lib().octhip_hamiltonian_destroy(self.handle)
+            try:
+                _lib.lib().octhip_hamiltonian_destroy(self.handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
             self.handle = None
 
 
@@ -196,6 +202,39 @@ class Exponential:
             raise ValueError("exponential_apply_batch needs a complex batch")
         call("octhip_exponential_taylor_batch", hm.handle, psib.ptr(), psib.ldf, float(deltat),
              self.exp_order, int(bool(imag_time)), _stream())
+
+
+class TaylorGraph:
+    """One Taylor step over several batches captured into a HIP graph: the loop over
+    batches of propagation_ops_elec_exp_apply (src/td/propagation_ops_elec.F90:232-249),
+    replayed with one launch per time step.  The batches must stay alive (and at the same
+    device address) for the life of the graph."""
+
+    def __init__(self, te: Exponential, hm: HamiltonianDevice, batches: list[Batch], deltat: float,
+                 imag_time: bool = False):
+        if not batches:
+            raise ValueError("TaylorGraph needs at least one batch")
+        ldf = batches[0].ldf
+        for b in batches:
+            if not b.is_complex or b.ldf != ldf:
+                raise ValueError("TaylorGraph batches must be complex with one ldf")
+        self.hm, self.batches = hm, list(batches)
+        ptrs = (C.c_void_p * len(batches))(*[b.ptr() for b in batches])
+        h = C.c_void_p()
+        call("octhip_taylor_graph_create", hm.handle, ptrs, len(batches), ldf, float(deltat),
+             te.exp_order, int(bool(imag_time)), _stream(), C.byref(h))
+        self.handle = h
+
+    def launch(self) -> None:
+        call("octhip_taylor_graph_launch", self.handle, _stream())
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _lib.lib().octhip_taylor_graph_destroy(self.handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
+            self.handle = None
 
 
 def exponential_apply_batch(te: Exponential, hm: HamiltonianDevice, psib: Batch, deltat: float,
