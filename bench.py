@@ -6,7 +6,7 @@ H = -1/2 Laplacian + v_loc + V_nl on the sphere R = 12 bohr, h = 0.25 (462,781 p
 order-4 star stencil, 64 states in two packed batches of 32, 16 Gaussian wells,
 16 atoms x 4 Kleinman-Bylander projectors.  One step = H applied to both batches.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor|adjoint]
 
 Prints one JSON line.  `value` is state-points/s with the states already resident in
 HBM; `roofline` is the dominant kernel (k_apply, the fused stencil + local + ket pass)
@@ -222,12 +222,77 @@ def bench_taylor(args, torch, api, _lib, workloads):
                               for k, v in kern.items()}}
 
 
+def bench_adjoint(args, torch, api, _lib, workloads):
+    """100 Taylor-4 steps (dt 0.02) on the R=10 h=0.3 sphere with 32 states and a dipole
+    field, forward with checkpoints every 10 steps, then the adjoint (dJ/deps, dJ/dv)."""
+    from octhip.autograd import overlap_objective
+    log("building the R=10 h=0.3 sphere workload (adjoint)")
+    s = workloads.adjoint_system()
+    nsteps, dt, K = 100, 0.02, 10
+    z, tg, eps = workloads.adjoint_problem(s, 32, nsteps, dt)
+    vol = s.mesh.vol_pp
+    hm = api.HamiltonianDevice.from_system(s)
+    hm.tune(bra_mode=args.bra, **tuning(args))
+    adj = api.Adjoint(hm, s.mesh.x, [1.0, 0.0, 0.0], 6, nsteps, dt, vol, checkpoint_every=K)
+    psi0 = api.Batch.from_states(z, s.np_part)
+    phib = api.Batch.from_states(tg, s.np_part)
+    te = api.Exponential(4)
+
+    def plain():  # the forward propagation alone, no checkpoints
+        p = psi0.clone()
+        for e in eps:
+            hm.set_dipole_field([e, 0.0, 0.0])
+            te.apply_batch(hm, p, dt)
+
+    def fwd_bwd():
+        p = psi0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        adj.forward(p, eps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J, chi = overlap_objective(phib, p, s.np, vol)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        adj.backward(chi)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return t1 - t0, t3 - t2, float(J)
+
+    for _ in range(max(1, args.warmup)):
+        plain()
+        fwd_bwd()
+    torch.cuda.synchronize()
+    reps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    _lib.profiling_start()
+    f = [fwd_bwd() for _ in range(reps)]
+    _lib.profiling_stop()
+    t_f = min(x[0] for x in f)
+    t_b = min(x[1] for x in f)
+    acc_ms, acc_n = _lib.profiling_get("k_adj_accum")
+    step_ms = t_b * 1e3 / nsteps
+    return {"metric": "FP64 adjoint step time (Taylor-4 backward + gradient accumulation)",
+            "value": step_ms, "unit": "ms/step", "higher_is_better": False, "n_gpus": 1,
+            "steps": reps, "warmup": args.warmup, "ms_per_step": step_ms, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded states, targets, field)",
+            "config": {"workload": "adjoint of 100 Taylor-4 steps dt=0.02, sphere R=10 h=0.3 "
+                                   f"(np={s.np}), 32 states, dipole x*eps(t), checkpoints every 10"},
+            "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "plain_forward_ms": t_plain * 1e3,
+            "backward_over_forward_step": t_b / t_plain, "J": f[-1][2],
+            "k_adj_accum_avg_us": acc_ms * 1e3 / acc_n if acc_n else None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor"])
+    ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor", "adjoint"])
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 gather, 2 z-march")
     ap.add_argument("--xcd-chunk", type=int, default=None)
     ap.add_argument("--gather-tile", type=int, default=None)
@@ -264,7 +329,8 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"calibration copy: {args.copy_calibration_mib} MiB read + written")
         del src, dst
-    fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor}[args.workload]
+    fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor,
+          "adjoint": bench_adjoint}[args.workload]
     res = fn(args, torch, api, _lib, workloads)
     print(json.dumps(res), flush=True)
     return 0

@@ -56,6 +56,7 @@ extern "C" {
 typedef struct octhip_nl_operator_s* octhip_nl_operator_t;
 typedef struct octhip_hamiltonian_s* octhip_hamiltonian_t;
 typedef struct octhip_taylor_graph_s* octhip_taylor_graph_t;
+typedef struct octhip_adjoint_s* octhip_adjoint_t;
 
 /* ---------------------------------------------------------------- lifecycle --
  * Replaces accel_init / accel_end (src/basic/accel.F90:264-700, 1105-1175). */
@@ -173,6 +174,30 @@ int octhip_taylor_graph_create(octhip_hamiltonian_t hm, double* const* psi, int 
                                octhip_taylor_graph_t* graph);
 int octhip_taylor_graph_launch(octhip_taylor_graph_t graph, void* stream);
 int octhip_taylor_graph_destroy(octhip_taylor_graph_t graph);
+
+/* ---------------------------------------------------------------- adjoint --
+ * Differentiable Taylor propagation of one complex batch under a dipole field
+ * H_s = H + eps_s (pol . r) (the length-gauge laser term, src/hamiltonian/lasers.F90:823-853),
+ * psi_{s+1} = sum_{n<=order} (-i deltat H_s)^n / n! psi_s for s = 0..nsteps-1.
+ * Counterpart of the optimal-control forward/backward pair (src/opt_control/propagation.F90:
+ * 579-783 bwd_step_2, 1016-1103 update_field, 1108-1205 checkpoints), but exact for the
+ * discrete propagator: given chi_N with dJ = 2 Re <chi_N|d psi_N> (<a|b> = vol_pp sum
+ * conj(a) b), backward returns dJ/d eps_s (host, nsteps) and dJ/dv(r) (device, np) and
+ * leaves chi_0 = dJ/d psi_0* in chi.  Forward stores psi every checkpoint_every steps;
+ * backward recomputes each segment from its checkpoint (memory: ceil(nsteps/K) batches
+ * of np_part rows plus K*order batches of np rows).
+ * create uploads positions (3*np doubles, host) into the Hamiltonian; forward and
+ * backward leave the Hamiltonian's dipole field at the field of the last step touched. */
+int octhip_adjoint_create(octhip_hamiltonian_t hm, const double* positions, const double* pol,
+                          int ldf, int nsteps, double deltat, int order, int checkpoint_every,
+                          double vol_pp, octhip_adjoint_t* adj);
+/* psi (device, np_part rows): in psi_0, out psi_N.  eps: nsteps doubles (host). */
+int octhip_adjoint_forward(octhip_adjoint_t adj, double* psi, const double* eps, void* stream);
+/* chi (device, np_part rows): in chi_N, out chi_0.  dj_deps: host, nsteps.
+ * dj_dv: device, np (overwritten).  Synchronises the stream before returning. */
+int octhip_adjoint_backward(octhip_adjoint_t adj, double* chi, double* dj_deps, double* dj_dv,
+                            void* stream);
+int octhip_adjoint_destroy(octhip_adjoint_t adj);
 
 /* --------------------------------------------------------------- profiling --
  * Per-kernel device timing with HIP events recorded on the launch stream, the device-side

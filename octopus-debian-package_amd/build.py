@@ -5,7 +5,8 @@
 Every .hip file of csrc/ is compiled with hipcc for gfx950 and linked into
 lib/liboctopus_hip.so.  Floating-point contraction is off for host and device code:
 the kernels spell out every fused multiply-add with fma(), so results do not depend on
-the compiler's contraction choices.
+the compiler's contraction choices.  The Fortran iso_c_binding module (fortran/octhip_m.F90)
+and its host driver are then compiled with amdflang into lib/octhip_driver.
 """
 from __future__ import annotations
 
@@ -25,6 +26,9 @@ LIBNAME = "liboctopus_hip.so"
 OBJDIR = PKG / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("OCTHIP_FC", "/opt/rocm/bin/amdflang")
+FDIR = PKG / "fortran"
+DRIVER = "octhip_driver"
 ARCH = "gfx950"
 
 COMMON = [
@@ -62,7 +66,26 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")
+    build_fortran(verbose)
     return out
+
+
+def build_fortran(verbose: bool = False) -> Path | None:
+    """amdflang: the bind(C) module and the driver program linked to liboctopus_hip.so."""
+    if shutil.which(FC) is None and not Path(FC).exists():
+        if verbose:
+            print(f"{FC} not found: Fortran binding not built")
+        return None
+    mod = OBJDIR / "fmod"
+    mod.mkdir(parents=True, exist_ok=True)
+    exe = LIBDIR / DRIVER
+    tmp = LIBDIR / (DRIVER + ".tmp")
+    _run([FC, "-O2", f"-J{mod}", "-o", str(tmp), str(FDIR / "octhip_m.F90"), str(FDIR / "octhip_driver.F90"),
+          f"-L{LIBDIR}", "-loctopus_hip", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, exe)
+    if verbose:
+        print(f"built {exe}")
+    return exe
 
 
 if __name__ == "__main__":

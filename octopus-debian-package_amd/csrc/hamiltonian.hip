@@ -360,7 +360,7 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   return launch_operator(h->kin, a, EPI_STORE, s);
 }
 
-namespace {
+namespace octhip {
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf) {
   int rc = ensure_work(h, ldf);
   if (!rc) rc = ensure_taylor(h, ldf);
@@ -368,9 +368,12 @@ int prepare_taylor(octhip_hamiltonian_s* h, int ldf) {
   return rc;
 }
 
-int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                hipStream_t s);
-}  // namespace
+int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s) {
+  for (int d = 0; d < 3; ++d) h->field[d] = field[d];
+  h->has_field = true;
+  return update_veff(h, s);
+}
+}  // namespace octhip
 
 extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
                                                int order, int imag_time, void* stream) {
@@ -380,7 +383,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
   int rc = prepare_taylor(h, ldf);
   if (rc) return rc;
-  return taylor_impl(h, psi, ldf, deltat, order, imag_time, (hipStream_t)stream);
+  return taylor_step(h, psi, ldf, deltat, order, imag_time, nullptr, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- graphs --
@@ -413,7 +416,7 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
     return fail(OCTHIP_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
   }
   set_capturing(true);
-  for (int b = 0; b < nbatch && !rc; ++b) rc = taylor_impl(h, psi[b], ldf, deltat, order, imag_time, cs);
+  for (int b = 0; b < nbatch && !rc; ++b) rc = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs);
   set_capturing(false);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(cs, &graph);
@@ -451,13 +454,17 @@ extern "C" int octhip_taylor_graph_destroy(octhip_taylor_graph_t g) {
   return OCTHIP_SUCCESS;
 }
 
-namespace {
-int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                hipStream_t s) {
+namespace octhip {
+// One Taylor step in place.  save (optional, order pointers to np-row buffers) receives
+// save[0] = psi (boundary cleared) and save[k] = H^k psi for k = 1..order-1: the terms the
+// discrete adjoint contracts with the backward states.
+int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                double* const* save, hipStream_t s) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
   int rc = zero_boundary(psi, op->np, op->np_part, L, s);
   if (rc) return rc;
+  if (save) OCTHIP_HIP(hipMemcpyAsync(save[0], psi, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
   // zfact *= (-i * deltat) / k  (exponential.F90:720-724), deltat -> i*deltat in imaginary time
   const std::complex<double> dt_c = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
   const std::complex<double> mi(-0.0, -1.0);
@@ -468,8 +475,8 @@ int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
     const std::complex<double> zz(z.real() * step.real() - z.imag() * step.imag(),
                                   z.real() * step.imag() + z.imag() * step.real());
     z = std::complex<double>(zz.real() / k, zz.imag() / k);
-    const double* yin = (k == 1) ? psi : h->d_y[(k - 2) & 1];
-    double* yout = (k < order) ? h->d_y[(k - 1) & 1] : nullptr;
+    const double* yin = (k == 1) ? psi : (save ? save[k - 1] : h->d_y[(k - 2) & 1]);
+    double* yout = (k < order) ? (save ? save[k] : h->d_y[(k - 1) & 1]) : nullptr;
     rc = run_bra(h, yin, L, s);
     if (rc) return rc;
     ApplyArgs a;
@@ -506,4 +513,4 @@ int taylor_impl(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
   }
   return OCTHIP_SUCCESS;
 }
-}  // namespace
+}  // namespace octhip

@@ -159,3 +159,12 @@ struct octhip_hamiltonian_s {
   int y_ldf = -1;
   int use_mfma = 1;
 };
+
+namespace octhip {
+// Taylor building blocks shared by the propagator entry points and the adjoint.
+int prepare_taylor(octhip_hamiltonian_s* h, int ldf);
+int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                double* const* save, hipStream_t s);
+// field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
+int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
+}  // namespace octhip

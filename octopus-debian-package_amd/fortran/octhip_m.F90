@@ -1,0 +1,209 @@
+! iso_c_binding interface to liboctopus_hip.so (include/octhip.h).
+!
+! This is the module Octopus's Fortran would `use` in the BATCH_DEVICE_PACKED branches of
+! X(nl_operator_operate_batch) (src/grid/nl_operator_inc.F90:257-469),
+! X(hamiltonian_elec_apply_batch) (src/hamiltonian/hamiltonian_elec_inc.F90:77-261) and
+! exponential_taylor_series_batch (src/electrons/exponential.F90:677-757).  Device
+! buffers are type(c_ptr), as accel_mem_t%mem is on the CUDA build (src/basic/accel.F90:
+! 150-161); handles are type(c_ptr); every function returns the library's status code
+! (0 = success), which the caller turns into messages_fatal as the reference does with
+! its device errors.  Integer arguments are c_int and passed by value, except arrays.
+module octhip_m
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: &
+    OCTHIP_SUCCESS = 0, OCTHIP_ERR_INVALID = -1, OCTHIP_ERR_HIP = -2, OCTHIP_ERR_ALLOC = -3, &
+    OCTHIP_ERR_UNSUPPORTED = -4, OCTHIP_ERR_STATE = -5
+  integer(c_int), parameter, public :: OCTHIP_OP_INNER = 1, OCTHIP_OP_OUTER = 2, OCTHIP_OP_ALL = 3
+  integer(c_int), parameter, public :: &
+    OCTHIP_TERM_KINETIC = 1, OCTHIP_TERM_LOCAL_POTENTIAL = 2, OCTHIP_TERM_NON_LOCAL_POTENTIAL = 4, &
+    OCTHIP_TERM_ALL = int(z'7fffffff', c_int)
+
+  public :: octhip_init, octhip_finalize, octhip_get_last_error, octhip_version, &
+    octhip_malloc, octhip_free, octhip_memcpy_htod, octhip_memcpy_dtoh, octhip_stream_synchronize, &
+    octhip_nl_operator_create, octhip_nl_operator_destroy, octhip_nl_operator_operate_batch, &
+    octhip_operate_ri_vec, octhip_hamiltonian_create, octhip_hamiltonian_destroy, &
+    octhip_hamiltonian_set_potential, octhip_hamiltonian_set_dipole_field, &
+    octhip_hamiltonian_apply_batch, octhip_exponential_taylor_batch, &
+    octhip_taylor_graph_create, octhip_taylor_graph_launch, octhip_taylor_graph_destroy, &
+    octhip_adjoint_create, octhip_adjoint_forward, octhip_adjoint_backward, octhip_adjoint_destroy, &
+    octhip_last_error_message
+
+  interface
+    integer(c_int) function octhip_init(device) bind(C, name="octhip_init")
+      import :: c_int
+      integer(c_int), value :: device
+    end function
+    integer(c_int) function octhip_finalize() bind(C, name="octhip_finalize")
+      import :: c_int
+    end function
+    integer(c_int) function octhip_get_last_error(buf, len) bind(C, name="octhip_get_last_error")
+      import :: c_int, c_char, c_size_t
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_size_t), value :: len
+    end function
+    integer(c_int) function octhip_version(major, minor) bind(C, name="octhip_version")
+      import :: c_int
+      integer(c_int), intent(out) :: major, minor
+    end function
+    integer(c_int) function octhip_malloc(ptr, bytes) bind(C, name="octhip_malloc")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), intent(out) :: ptr
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_int) function octhip_free(ptr) bind(C, name="octhip_free")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ptr
+    end function
+    integer(c_int) function octhip_memcpy_htod(dst, src, bytes, stream) bind(C, name="octhip_memcpy_htod")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: dst, src, stream
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_int) function octhip_memcpy_dtoh(dst, src, bytes, stream) bind(C, name="octhip_memcpy_dtoh")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: dst, src, stream
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_int) function octhip_stream_synchronize(stream) bind(C, name="octhip_stream_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: stream
+    end function
+
+    ! nl_operator_build's device upload (src/grid/nl_operator.F90:516-582): w = op%w(:,1),
+    ! ri = op%ri (n x nri), rimap_inv = op%rimap_inv (0-based half-open runs), points =
+    ! op%stencil%points (3 x n) or c_null_ptr, coords = lattice coordinates or c_null_ptr
+    integer(c_int) function octhip_nl_operator_create(n, w, stencil_points, nri, ri, rimap_inv, &
+        np, np_part, coords, op) bind(C, name="octhip_nl_operator_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n, nri, np, np_part
+      real(c_double), intent(in) :: w(*)
+      type(c_ptr), value :: stencil_points, coords
+      integer(c_int), intent(in) :: ri(*), rimap_inv(*)
+      type(c_ptr), intent(out) :: op
+    end function
+    integer(c_int) function octhip_nl_operator_destroy(op) bind(C, name="octhip_nl_operator_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+    end function
+    ! X(nl_operator_operate_batch)(op, fi, fo, points=points, factor=factor) on packed
+    ! device batches; ldf = log2(pack_size_real(1))
+    integer(c_int) function octhip_nl_operator_operate_batch(op, fi, fo, ldf, factor, points, stream) &
+        bind(C, name="octhip_nl_operator_operate_batch")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: op, fi, fo, stream
+      integer(c_int), value :: ldf, points
+      real(c_double), value :: factor
+    end function
+    ! doperate_ri_vec/zoperate_ri_vec (src/grid/operate.c:40-76) on device arrays
+    integer(c_int) function octhip_operate_ri_vec(n, w, nri, ri, rimap_inv, rimap_inv_max, fi, ldf, fo, &
+        stream) bind(C, name="octhip_operate_ri_vec")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n, nri, ldf
+      real(c_double), intent(in) :: w(*)
+      type(c_ptr), value :: ri, rimap_inv, rimap_inv_max, fi, fo, stream
+    end function
+
+    ! build_accel's upload (src/hamiltonian/hamiltonian_elec_base.F90:755-909)
+    integer(c_int) function octhip_hamiltonian_create(kinetic, mass, vlocal, nmat, offsets, matrices, &
+        maps, scals, hm) bind(C, name="octhip_hamiltonian_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: kinetic, vlocal, offsets, matrices, maps, scals
+      real(c_double), value :: mass
+      integer(c_int), value :: nmat
+      type(c_ptr), intent(out) :: hm
+    end function
+    integer(c_int) function octhip_hamiltonian_destroy(hm) bind(C, name="octhip_hamiltonian_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm
+    end function
+    integer(c_int) function octhip_hamiltonian_set_potential(hm, vlocal, stream) &
+        bind(C, name="octhip_hamiltonian_set_potential")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, stream
+      real(c_double), intent(in) :: vlocal(*)
+    end function
+    integer(c_int) function octhip_hamiltonian_set_dipole_field(hm, positions, field, stream) &
+        bind(C, name="octhip_hamiltonian_set_dipole_field")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, positions, stream
+      real(c_double), intent(in) :: field(3)
+    end function
+    ! X(hamiltonian_elec_apply_batch)(hm, mesh, psib, hpsib, terms, set_bc)
+    integer(c_int) function octhip_hamiltonian_apply_batch(hm, psi, hpsi, ldf, is_complex, terms, set_bc, &
+        stream) bind(C, name="octhip_hamiltonian_apply_batch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, psi, hpsi, stream
+      integer(c_int), value :: ldf, is_complex, terms, set_bc
+    end function
+    ! exponential_apply_batch(te, ..., psib, deltat, imag_time=imag_time), Taylor
+    integer(c_int) function octhip_exponential_taylor_batch(hm, psi, ldf, deltat, order, imag_time, stream) &
+        bind(C, name="octhip_exponential_taylor_batch")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, psi, stream
+      integer(c_int), value :: ldf, order, imag_time
+      real(c_double), value :: deltat
+    end function
+    integer(c_int) function octhip_taylor_graph_create(hm, psi, nbatch, ldf, deltat, order, imag_time, &
+        stream, graph) bind(C, name="octhip_taylor_graph_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, stream
+      type(c_ptr), intent(in) :: psi(*)
+      integer(c_int), value :: nbatch, ldf, order, imag_time
+      real(c_double), value :: deltat
+      type(c_ptr), intent(out) :: graph
+    end function
+    integer(c_int) function octhip_taylor_graph_launch(graph, stream) bind(C, name="octhip_taylor_graph_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: graph, stream
+    end function
+    integer(c_int) function octhip_taylor_graph_destroy(graph) bind(C, name="octhip_taylor_graph_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: graph
+    end function
+
+    integer(c_int) function octhip_adjoint_create(hm, positions, pol, ldf, nsteps, deltat, order, &
+        checkpoint_every, vol_pp, adj) bind(C, name="octhip_adjoint_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm
+      real(c_double), intent(in) :: positions(*), pol(3)
+      integer(c_int), value :: ldf, nsteps, order, checkpoint_every
+      real(c_double), value :: deltat, vol_pp
+      type(c_ptr), intent(out) :: adj
+    end function
+    integer(c_int) function octhip_adjoint_forward(adj, psi, eps, stream) bind(C, name="octhip_adjoint_forward")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: adj, psi, stream
+      real(c_double), intent(in) :: eps(*)
+    end function
+    integer(c_int) function octhip_adjoint_backward(adj, chi, dj_deps, dj_dv, stream) &
+        bind(C, name="octhip_adjoint_backward")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: adj, chi, dj_dv, stream
+      real(c_double), intent(out) :: dj_deps(*)
+    end function
+    integer(c_int) function octhip_adjoint_destroy(adj) bind(C, name="octhip_adjoint_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: adj
+    end function
+  end interface
+
+contains
+
+  ! The last error message as a Fortran string (for messages_fatal).
+  function octhip_last_error_message() result(msg)
+    character(len=1024) :: msg
+    character(kind=c_char) :: buf(1024)
+    integer :: i, ierr
+    buf = c_null_char
+    ierr = octhip_get_last_error(buf, int(size(buf), c_size_t))
+    msg = ''
+    do i = 1, size(buf)
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+  end function
+
+end module octhip_m

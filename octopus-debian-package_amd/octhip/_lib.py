@@ -50,6 +50,10 @@ SIGNATURES = {
     "octhip_taylor_graph_create": [_P, _P, _I, _I, _D, _I, _I, _P, _P],
     "octhip_taylor_graph_launch": [_P, _P],
     "octhip_taylor_graph_destroy": [_P],
+    "octhip_adjoint_create": [_P, _P, _P, _I, _I, _D, _I, _I, _D, _P],
+    "octhip_adjoint_forward": [_P, _P, _P, _P],
+    "octhip_adjoint_backward": [_P, _P, _P, _P, _P],
+    "octhip_adjoint_destroy": [_P],
     "octhip_profiling_start": [],
     "octhip_profiling_stop": [],
     "octhip_profiling_get": [C.c_char_p, _DP, _IP],

@@ -240,3 +240,44 @@ class TaylorGraph:
 def exponential_apply_batch(te: Exponential, hm: HamiltonianDevice, psib: Batch, deltat: float,
                             imag_time: bool = False) -> None:
     te.apply_batch(hm, psib, deltat, imag_time)
+
+
+class Adjoint:
+    """Differentiable Taylor propagation of one complex batch under a dipole field
+    H_s = H + eps_s (pol . r): the forward/backward pair of the optimal-control gradient
+    (src/opt_control/propagation.F90:579-783, 1016-1103) made exact for the discrete
+    propagator, with checkpoints every ``checkpoint_every`` steps (1108-1205)."""
+
+    def __init__(self, hm: HamiltonianDevice, positions: np.ndarray, pol, ldf: int, nsteps: int, deltat: float,
+                 vol_pp: float, order: int = 4, checkpoint_every: int = 10):
+        x = np.ascontiguousarray(positions, dtype=np.float64)[: hm.np].ravel()
+        p = np.ascontiguousarray(pol, dtype=np.float64)
+        h = C.c_void_p()
+        call("octhip_adjoint_create", hm.handle, as_ptr(x, C.c_double), as_ptr(p, C.c_double), int(ldf),
+             int(nsteps), float(deltat), int(order), int(checkpoint_every), float(vol_pp), C.byref(h))
+        self.handle, self.hm, self.nsteps, self.ldf = h, hm, int(nsteps), int(ldf)
+
+    def forward(self, psib: Batch, eps) -> None:
+        """psib: psi_0 in, psi_N out (in place)."""
+        e = np.ascontiguousarray(eps, dtype=np.float64)
+        if e.shape != (self.nsteps,) or psib.ldf != self.ldf:
+            raise ValueError("eps must hold one value per step and the batch must match ldf")
+        call("octhip_adjoint_forward", self.handle, psib.ptr(), as_ptr(e, C.c_double), _stream())
+
+    def backward(self, chib: Batch):
+        """chib: chi_N in, chi_0 out (in place).  Returns (dJ/deps numpy (nsteps,),
+        dJ/dv torch float64 (np,) on the device)."""
+        torch = _torch()
+        deps = np.zeros(self.nsteps)
+        dv = torch.empty(self.hm.np, dtype=torch.float64, device="cuda")
+        call("octhip_adjoint_backward", self.handle, chib.ptr(), as_ptr(deps, C.c_double),
+             C.c_void_p(dv.data_ptr()), _stream())
+        return deps, dv
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _lib.lib().octhip_adjoint_destroy(self.handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
+            self.handle = None

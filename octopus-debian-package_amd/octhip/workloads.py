@@ -170,3 +170,28 @@ def dipole_field(seed: int, nsin: int = 5):
 def eps_of_t(t, w, A, phi):
     t = np.asarray(t, dtype=np.float64)
     return np.sum(A[None, :] * np.sin(w[None, :] * t[..., None] + phi[None, :]), axis=-1)
+
+
+def adjoint_system(seed=None) -> System:
+    """Sphere R=10, h=0.3 with the sphere-R12 Hamiltonian recipe (16 wells, 16 KB atoms)."""
+    seed = rng.SEED_BASE + 4 if seed is None else seed
+    return hamiltonian_system(radius=10.0, spacing=0.3, seed=seed, name="adjoint_sphere")
+
+
+def adjoint_problem(system: System, nst: int = 32, nsteps: int = 100, dt: float = 0.02):
+    """Initial states, targets (both orthonormalised, seeded) and eps at the step
+    midpoints t_s + dt/2 (exponential-midpoint convention)."""
+    vol = system.mesh.vol_pp
+    psi0 = orthonormalise(gaussian_envelope_states(system, nst), vol)
+    m = system.mesh
+    x = m.x[: m.np]
+    env = np.exp(-np.einsum("ij,ij->i", x, x) / 4.0 ** 2)
+    tg = np.empty((nst, m.np), dtype=np.complex128)
+    for i in range(nst):
+        re = rng.normal(system.seed, _S["targets"] * 100000 + 2 * i, m.np)
+        im = rng.normal(system.seed, _S["targets"] * 100000 + 2 * i + 1, m.np)
+        tg[i] = (re + 1j * im) * env
+    phi = orthonormalise(tg, vol)
+    w, A, ph = dipole_field(system.seed)
+    eps = eps_of_t((np.arange(nsteps) + 0.5) * dt, w, A, ph)
+    return psi0, phi, eps
