@@ -57,6 +57,7 @@ typedef struct octhip_nl_operator_s* octhip_nl_operator_t;
 typedef struct octhip_hamiltonian_s* octhip_hamiltonian_t;
 typedef struct octhip_taylor_graph_s* octhip_taylor_graph_t;
 typedef struct octhip_adjoint_s* octhip_adjoint_t;
+typedef struct octhip_batched_s* octhip_batched_t;
 
 /* ---------------------------------------------------------------- lifecycle --
  * Replaces accel_init / accel_end (src/basic/accel.F90:264-700, 1105-1175). */
@@ -174,6 +175,27 @@ int octhip_taylor_graph_create(octhip_hamiltonian_t hm, double* const* psi, int 
                                octhip_taylor_graph_t* graph);
 int octhip_taylor_graph_launch(octhip_taylor_graph_t graph, void* stream);
 int octhip_taylor_graph_destroy(octhip_taylor_graph_t graph);
+
+/* ---------------------------------------------------------------- batched --
+ * Several independent (Hamiltonian, batch) pairs -- systems of uneven size, several
+ * batches per system -- propagated together: every kernel kind of a Taylor order runs
+ * once for all pairs (a work list of (pair, tile segment, lane chunk) for the fused
+ * stencil pass, block-offset tables for the projector kernels), instead of one launch
+ * sequence per batch and system as in the reference's loops (propagation_ops_elec_exp_apply,
+ * src/td/propagation_ops_elec.F90:232-249, per system).  Results are bit-identical to
+ * octhip_exponential_taylor_batch / octhip_hamiltonian_apply_batch per pair.
+ * hms[i], psi[i] (device, np_part rows), ldf[i] (complex batches), hpsi[i] (device, np
+ * rows; the whole array may be NULL if octhip_batched_apply is not used).  Every pair
+ * needs the order-4 star on a lattice; pairs may share a Hamiltonian (each pair has its
+ * own work buffers).  order in 2..32.  The pointers are fixed at create. */
+int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms, double* const* psi,
+                          double* const* hpsi, const int* ldf, double deltat, int order,
+                          octhip_batched_t* batched);
+/* hpsi[i] = H_i psi[i] for every pair (zero boundary, every term) */
+int octhip_batched_apply(octhip_batched_t batched, void* stream);
+/* psi[i] <- sum_k (-i deltat H_i)^k / k! psi[i] for every pair */
+int octhip_batched_taylor_step(octhip_batched_t batched, void* stream);
+int octhip_batched_destroy(octhip_batched_t batched);
 
 /* ---------------------------------------------------------------- adjoint --
  * Differentiable Taylor propagation of one complex batch under a dipole field

@@ -361,6 +361,35 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
 }
 
 namespace octhip {
+// zfact_k, k = 1..order: zfact *= (-i * deltat) / k (exponential.F90:720-724); deltat ->
+// i*deltat in imaginary time (609-619).  z[0] = 1.
+void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z) {
+  const std::complex<double> dt_c = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
+  const std::complex<double> mi(-0.0, -1.0);
+  const std::complex<double> step(mi.real() * dt_c.real() - mi.imag() * dt_c.imag(),
+                                  mi.real() * dt_c.imag() + mi.imag() * dt_c.real());
+  z[0] = std::complex<double>(1.0, 0.0);
+  for (int k = 1; k <= order; ++k) {
+    const std::complex<double> p = z[k - 1];
+    const std::complex<double> zz(p.real() * step.real() - p.imag() * step.imag(),
+                                  p.real() * step.imag() + p.imag() * step.real());
+    z[k] = std::complex<double>(zz.real() / k, zz.imag() / k);
+  }
+}
+
+// Arguments of the fused pass H `in` for every term, boundary rows of `in` read as zero,
+// ket rows taken from `ksum` (a work buffer of the caller's choice).
+void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a) {
+  std::memset(&a, 0, sizeof(a));
+  fill_terms(h, OCTHIP_TERM_ALL, a);
+  a.fi = in;
+  a.L = L;
+  a.lim = h->kin->np;
+  // fill_terms points ksum at the handle's own buffer, which may not exist yet: the
+  // caller's buffer replaces it whenever the nonlocal rows are used
+  if (a.kslot) a.ksum = ksum;
+}
+
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf) {
   int rc = ensure_work(h, ldf);
   if (!rc) rc = ensure_taylor(h, ldf);
@@ -465,16 +494,11 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
   int rc = zero_boundary(psi, op->np, op->np_part, L, s);
   if (rc) return rc;
   if (save) OCTHIP_HIP(hipMemcpyAsync(save[0], psi, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
-  // zfact *= (-i * deltat) / k  (exponential.F90:720-724), deltat -> i*deltat in imaginary time
-  const std::complex<double> dt_c = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
-  const std::complex<double> mi(-0.0, -1.0);
-  const std::complex<double> step(mi.real() * dt_c.real() - mi.imag() * dt_c.imag(),
-                                  mi.real() * dt_c.imag() + mi.imag() * dt_c.real());
-  std::complex<double> z(1.0, 0.0), z1;
+  std::complex<double> zf[33];
+  taylor_coefs(deltat, imag_time, order, zf);
+  std::complex<double> z, z1;
   for (int k = 1; k <= order; ++k) {
-    const std::complex<double> zz(z.real() * step.real() - z.imag() * step.imag(),
-                                  z.real() * step.imag() + z.imag() * step.real());
-    z = std::complex<double>(zz.real() / k, zz.imag() / k);
+    z = zf[k];
     const double* yin = (k == 1) ? psi : (save ? save[k - 1] : h->d_y[(k - 2) & 1]);
     double* yout = (k < order) ? (save ? save[k] : h->d_y[(k - 1) & 1]) : nullptr;
     rc = run_bra(h, yin, L, s);

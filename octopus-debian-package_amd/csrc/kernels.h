@@ -274,6 +274,23 @@ struct KetArgs {
   int nmulti, total_points;
 };
 
+struct BraMfmaArgs {
+  BraArgs b;
+  const double* __restrict__ pt;  // total_points x pjs
+  int pjs;
+};
+
+struct ZeroArgs {
+  double* f;
+  size_t first, count;            // zero f[first .. first+count)
+};
+
+// kernels of a multi-segment launch (several systems/batches in one grid)
+enum { MULTI_BRA_MFMA = 1, MULTI_BRA_PLAIN, MULTI_BRA_REDUCE, MULTI_KET_MFMA, MULTI_KET_PLAIN,
+       MULTI_KET_SUM, MULTI_ZERO };
+// args: device array of nseg argument blocks; off: device array of nseg+1 block offsets
+int launch_multi(int kind, const void* args, const int* off, int nseg, int nblocks, hipStream_t s);
+
 int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s);
 int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s);
 int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s);

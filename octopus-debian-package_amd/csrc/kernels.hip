@@ -64,9 +64,8 @@ int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s) {
 // slices are then added in fixed order through LDS, so results are deterministic.
 constexpr int BRA_PJB = 8;  // projectors per register block
 
-__global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) {
+__device__ __forceinline__ void bra_plain_body(const BraArgs& b, int c) {
   __shared__ double red[BLOCK * BRA_PJB];
-  const int c = blockIdx.x;
   const int4 ch = b.chunks[c];
   const int4 mi = b.mat_info[ch.x];
   const int npt = mi.x, npj = mi.y, moff = mi.z, mapoff = mi.w;
@@ -102,6 +101,8 @@ __global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) {
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) { bra_plain_body(b, blockIdx.x); }
+
 // --------------------------------------------------------------- bra (MFMA) --
 // D(16 projectors x 16 lanes) += A(16 x 4 sphere points) * B(4 x 16 lanes) with
 // v_mfma_f64_16x16x4_f64.  Operand layout (wave64): lane l supplies A[l%16][l/16] and
@@ -111,17 +112,11 @@ __global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) {
 // sphere point's row), B from the state rows (16 lanes read 128 contiguous bytes).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-struct BraMfmaArgs {
-  BraArgs b;
-  const double* __restrict__ pt;  // total_points x pjs
-  int pjs;
-};
 
-__global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
+__device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c) {
   constexpr int KB = 16;                    // MFMA steps (4 sphere points each) per load block
   __shared__ double red[4 * 16 * 16];
   const BraArgs& b = m.b;
-  const int c = blockIdx.x;
   const int4 ch = b.chunks[c];
   const int4 mi = b.mat_info[ch.x];
   const int npj = mi.y, mapoff = mi.w;
@@ -185,11 +180,12 @@ __global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) { bra_mfma_body(m, blockIdx.x); }
+
 // proj[i, l] = scal[i] * sum_{chunks of matrix(i)} partial[chunk, i - pbase, l]
-__global__ void k_bra_reduce(const BraReduceArgs r) {
+__device__ __forceinline__ void bra_reduce_body(const BraReduceArgs& r, int bid, int nblk) {
   const size_t total = (size_t)r.nproj * r.L;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
+  for (size_t t = bid * (size_t)BLOCK + threadIdx.x; t < total; t += (size_t)nblk * BLOCK) {
     const int i = (int)(t / r.L);
     const int l = (int)(t - (size_t)i * r.L);
     const int a = r.proj_mat[i];
@@ -206,6 +202,10 @@ __global__ void k_bra_reduce(const BraReduceArgs r) {
     }
     r.proj[t] = s * r.scals[i];
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bra_reduce(const BraReduceArgs r) {
+  bra_reduce_body(r, blockIdx.x, gridDim.x);
 }
 
 }  // namespace octhip
@@ -236,8 +236,8 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
 // D(16 points x 16 lanes) = A(16 points x 4 projectors) B(4 projectors x 16 lanes); lane
 // (kk, ii) supplies A[ii][kk] = P[q0+ii, jb+kk] and B[kk][ii] = proj[pbase+jb+kk, col];
 // accumulator element r of lane (kk, ii) is D[4r+kk][ii].
-__global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) {
-  const int4 ch = k.chunks[blockIdx.x];
+__device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c) {
+  const int4 ch = k.chunks[c];
   const int4 mi = k.mat_info[ch.x];
   const int npj = mi.y, mapoff = mi.w;
   const int pb = k.mat_pbase[ch.x];
@@ -264,9 +264,11 @@ __global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) {
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) { ket_mfma_body(k, blockIdx.x); }
+
 // plain form, any row length: one thread per (sphere point, lane)
-__global__ __launch_bounds__(BLOCK) void k_ket_plain(const KetArgs k) {
-  const int4 ch = k.chunks[blockIdx.x];
+__device__ __forceinline__ void ket_plain_body(const KetArgs& k, int c) {
+  const int4 ch = k.chunks[c];
   const int4 mi = k.mat_info[ch.x];
   const int npj = mi.y, mapoff = mi.w;
   const int pb = k.mat_pbase[ch.x];
@@ -280,10 +282,9 @@ __global__ __launch_bounds__(BLOCK) void k_ket_plain(const KetArgs k) {
   }
 }
 
-__global__ void k_ket_sum(const KetArgs k) {
+__device__ __forceinline__ void ket_sum_body(const KetArgs& k, int bid, int nblk) {
   const size_t total = (size_t)k.nmulti * k.L;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
-       t += (size_t)gridDim.x * blockDim.x) {
+  for (size_t t = bid * (size_t)BLOCK + threadIdx.x; t < total; t += (size_t)nblk * BLOCK) {
     const int m = (int)(t / k.L);
     const int l = (int)(t - (size_t)m * k.L);
     double s = 0.0;
@@ -291,6 +292,9 @@ __global__ void k_ket_sum(const KetArgs k) {
     k.kbuf[((size_t)k.total_points + m) * k.L + l] = s;
   }
 }
+
+__global__ __launch_bounds__(BLOCK) void k_ket_plain(const KetArgs k) { ket_plain_body(k, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_ket_sum(const KetArgs k) { ket_sum_body(k, blockIdx.x, gridDim.x); }
 
 int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
   if (nchunk <= 0) return OCTHIP_SUCCESS;
@@ -325,4 +329,112 @@ int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s) {
   return OCTHIP_SUCCESS;
 }
 
+}  // namespace octhip
+
+// ------------------------------------------------------- multi-segment launches --
+// One launch over several (system, batch) pairs: segment i owns blocks off[i] ..
+// off[i+1]-1 and its argument block args[i]; a block finds its segment by bisection over
+// the (uniform) offset table.  Empty segments are skipped by taking the last segment
+// whose first block is <= blockIdx.
+namespace octhip {
+namespace {
+__device__ __forceinline__ int segment_of(const int* __restrict__ off, int nseg, int b) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bra_mfma_multi(const BraMfmaArgs* __restrict__ a,
+                                                          const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  bra_mfma_body(a[i], blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_bra_plain_multi(const BraMfmaArgs* __restrict__ a,
+                                                           const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  bra_plain_body(a[i].b, blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_bra_reduce_multi(const BraReduceArgs* __restrict__ a,
+                                                            const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  bra_reduce_body(a[i], blockIdx.x - off[i], off[i + 1] - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_ket_mfma_multi(const KetArgs* __restrict__ a,
+                                                          const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  ket_mfma_body(a[i], blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_ket_plain_multi(const KetArgs* __restrict__ a,
+                                                           const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  ket_plain_body(a[i], blockIdx.x - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_ket_sum_multi(const KetArgs* __restrict__ a,
+                                                         const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  ket_sum_body(a[i], blockIdx.x - off[i], off[i + 1] - off[i]);
+}
+__global__ __launch_bounds__(BLOCK) void k_zero_multi(const ZeroArgs* __restrict__ a,
+                                                      const int* __restrict__ off, int nseg) {
+  const int i = segment_of(off, nseg, blockIdx.x);
+  const ZeroArgs z = a[i];
+  const size_t nb = (size_t)(off[i + 1] - off[i]);
+  for (size_t t = (blockIdx.x - off[i]) * (size_t)BLOCK + threadIdx.x; t < z.count; t += nb * BLOCK)
+    z.f[z.first + t] = 0.0;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "%s launch: %s", what, hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+int launch_multi(int kind, const void* args, const int* off, int nseg, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return OCTHIP_SUCCESS;
+  const dim3 g(nblocks), b(BLOCK);
+  switch (kind) {
+    case MULTI_BRA_MFMA: {
+      ProfScope prof("k_bra", s);
+      hipLaunchKernelGGL(k_bra_mfma_multi, g, b, 0, s, (const BraMfmaArgs*)args, off, nseg);
+      return check_launch("k_bra_mfma_multi");
+    }
+    case MULTI_BRA_PLAIN: {
+      ProfScope prof("k_bra", s);
+      hipLaunchKernelGGL(k_bra_plain_multi, g, b, 0, s, (const BraMfmaArgs*)args, off, nseg);
+      return check_launch("k_bra_plain_multi");
+    }
+    case MULTI_BRA_REDUCE: {
+      ProfScope prof("k_bra_reduce", s);
+      hipLaunchKernelGGL(k_bra_reduce_multi, g, b, 0, s, (const BraReduceArgs*)args, off, nseg);
+      return check_launch("k_bra_reduce_multi");
+    }
+    case MULTI_KET_MFMA: {
+      ProfScope prof("k_ket", s);
+      hipLaunchKernelGGL(k_ket_mfma_multi, g, b, 0, s, (const KetArgs*)args, off, nseg);
+      return check_launch("k_ket_mfma_multi");
+    }
+    case MULTI_KET_PLAIN: {
+      ProfScope prof("k_ket", s);
+      hipLaunchKernelGGL(k_ket_plain_multi, g, b, 0, s, (const KetArgs*)args, off, nseg);
+      return check_launch("k_ket_plain_multi");
+    }
+    case MULTI_KET_SUM: {
+      ProfScope prof("k_ket_sum", s);
+      hipLaunchKernelGGL(k_ket_sum_multi, g, b, 0, s, (const KetArgs*)args, off, nseg);
+      return check_launch("k_ket_sum_multi");
+    }
+    case MULTI_ZERO: {
+      ProfScope prof("k_zero_rows", s);
+      hipLaunchKernelGGL(k_zero_multi, g, b, 0, s, (const ZeroArgs*)args, off, nseg);
+      return check_launch("k_zero_multi");
+    }
+    default:
+      return fail(OCTHIP_ERR_INVALID, "unknown multi-segment kernel %d", kind);
+  }
+}
 }  // namespace octhip

@@ -231,6 +231,8 @@ int prepare_operator(octhip_nl_operator_s* op, int L) {
 // Apply through the best kernel for this operator: z-march for the order-4 star on a
 // lattice, the tiled gather otherwise.
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s) {
+  if (!a.fi || !a.zrow || (a.kslot && !a.ksum) || (epi == EPI_STORE && !a.fo))
+    return fail(OCTHIP_ERR_STATE, "apply: incomplete kernel arguments");
   const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25;
   if (!zm) return launch_apply(a, op->lattice && a.n > 0, epi, s);
   int rc = zmarch_items(op);

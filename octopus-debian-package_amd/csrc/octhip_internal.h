@@ -1,6 +1,8 @@
 // Internal declarations shared by the HIP translation units of liboctopus_hip.so.
 #pragma once
 
+#include <complex>
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -165,6 +167,7 @@ namespace octhip {
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf);
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
                 double* const* save, hipStream_t s);
+void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
 int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
 }  // namespace octhip

@@ -35,8 +35,11 @@ struct ZArgs {
   const int2* __restrict__ sched;    // blockIdx -> {item, lane chunk}
 };
 
+// One work item: tile it = {tile x, tile y, first plane, end plane}, lane chunk `chunk`.
+// za may live in kernel arguments (single operator) or in global memory (one descriptor
+// per system of a batched launch): every field is copied to a local once.
 template <int LPT, int LC, int EPI, bool NL>
-__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
+__device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, const int chunk) {
   constexpr int PAIRS = LC / LPT;    // threads per column
   constexpr int NT = 64 * PAIRS;
   extern __shared__ double2 zsh2[];  // 16-byte aligned: row chunks move as ds_*_b128
@@ -44,18 +47,29 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
   double* const lds0 = zsh;                              // plane buffer 0
   double* const lds1 = zsh + ZS * LC;                    // plane buffer 1
   int* const tidx = reinterpret_cast<int*>(zsh + 2 * ZS * LC);  // item index table
-  const ApplyArgs& a = za.a;
-  const int2 sc = za.sched[blockIdx.x];
-  if (sc.x < 0) return;              // padding entry of the schedule (whole workgroup)
-  const int4 it = za.items[sc.x];
+  const ApplyArgs& A = za.a;
+  const double* const a_fi = A.fi;
+  double* const a_fo = A.fo;
+  const double* const a_zrow = A.zrow;
+  const int a_np = A.np, a_lim = A.lim;
+  const double* const a_v = A.v;
+  const int* const a_kslot = A.kslot;
+  const double* const a_ksum = A.ksum;
+  double* const a_yout = A.yout;
+  double* const a_acc = A.acc;
+  const double a_c1r = A.c1r, a_c1i = A.c1i, a_c2r = A.c2r, a_c2i = A.c2i;
+  const int a_has_c1 = A.has_c1;
+  double a_w[25];
+#pragma unroll
+  for (int j = 0; j < 25; ++j) a_w[j] = A.w[j];
   const int kp = threadIdx.x % PAIRS;
   const int col = threadIdx.x / PAIRS;
   const int cx = col & 7, cy = col >> 3;
-  const int l = sc.y * LC + kp * LPT;          // first lane of this thread
+  const int l = chunk * LC + kp * LPT;         // first lane of this thread
   const int x0 = it.x * ZT + ZR, y0 = it.y * ZT + ZR;
   const long mxl = za.mx;
   const long sxy = mxl * za.my;
-  const size_t L = (size_t)a.L;
+  const size_t L = (size_t)A.L;
   const int* __restrict__ mc = za.mcube;
   // two halo units per thread: h = col + 64*i, same lane pair as the thread
   int hlds[2];
@@ -101,7 +115,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
 
   // value of point q (zero row for boundary/non-points when q >= lim): branch-free load
   auto val = [&](int q) {
-    const double* src = ((unsigned)q < (unsigned)a.lim) ? a.fi + (size_t)q * L + l : a.zrow;
+    const double* src = ((unsigned)q < (unsigned)a_lim) ? a_fi + (size_t)q * L + l : a_zrow;
     return ldv<LPT>(src);
   };
   auto lds_st = [&](int buf, int off, const Vec<LPT>& v) {
@@ -161,7 +175,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
           kq[u] = -1;
           if (e < totk) {
             const int q = tidx[(e >> 6) * 192 + (e & 63)];
-            if ((unsigned)q < (unsigned)a.np) kq[u] = a.kslot[q];
+            if ((unsigned)q < (unsigned)a_np) kq[u] = a_kslot[q];
           }
         }
 #pragma unroll
@@ -187,18 +201,18 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
   double vr[4] = {0.0, 0.0, 0.0, 0.0};
   Vec<LPT> kr[4];
   Vec<LPT> ar[4];
-  const bool has_acc = (EPI == EPI_TAYLOR) && a.acc != nullptr;
+  const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
   // per-point data of plane P's output point: potential, ket sum row, accumulator row
   auto load_aux = [&](int slot, int P) {
     const int q = own_idx(P);
-    if ((unsigned)q < (unsigned)a.np) {
-      if (a.v) vr[slot] = a.v[q];
+    if ((unsigned)q < (unsigned)a_np) {
+      if (a_v) vr[slot] = a_v[q];
       if constexpr (NL) {
         const int ks = tks[(P - zb + 4) * 64 + col];
-        const double* src = ks >= 0 ? a.ksum + (size_t)ks * L + l : a.zrow;
+        const double* src = ks >= 0 ? a_ksum + (size_t)ks * L + l : a_zrow;
         kr[slot] = ldv<LPT>(src);
       }
-      if (has_acc) ar[slot] = ldv<LPT>(a.acc + (size_t)q * L + l);
+      if (has_acc) ar[slot] = ldv<LPT>(a_acc + (size_t)q * L + l);
     }
   };
 #pragma unroll
@@ -231,7 +245,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
       __syncthreads();
       // output point (x0+cx, y0+cy, z)
       const int q0 = own_idx(z);
-      if ((unsigned)q0 < (unsigned)a.np) {
+      if ((unsigned)q0 < (unsigned)a_np) {
         Vec<LPT> s;
 #pragma unroll
         for (int k = 0; k < LPT; ++k) s.v[k] = 0.0;
@@ -249,10 +263,10 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
             x = ov[(kk + 4 + dz + RQ) % RQ];
           }
 #pragma unroll
-          for (int k = 0; k < LPT; ++k) s.v[k] = fma(a.w[j], x.v[k], s.v[k]);
+          for (int k = 0; k < LPT; ++k) s.v[k] = fma(a_w[j], x.v[k], s.v[k]);
         }
         const Vec<LPT> x = ov[(kk + 4) % RQ];
-        if (a.v) {
+        if (a_v) {
           const double vv = vr[kk & 3];
 #pragma unroll
           for (int k = 0; k < LPT; ++k) s.v[k] = fma(vv, x.v[k], s.v[k]);
@@ -262,30 +276,48 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
           for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kr[kk & 3].v[k];
         }
         if constexpr (EPI == EPI_STORE) {
-          stv<LPT>(a.fo + (size_t)q0 * L + l, s);
+          stv<LPT>(a_fo + (size_t)q0 * L + l, s);
         } else {
-          if (a.yout) stv<LPT>(a.yout + (size_t)q0 * L + l, s);
+          if (a_yout) stv<LPT>(a_yout + (size_t)q0 * L + l, s);
           if (has_acc) {
             Vec<LPT> av = ar[kk & 3];
             if constexpr (LPT == 2) {
-              if (a.has_c1) {
-                av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], x.v[1], false);
-                av.v[1] = av.v[1] + zmul_lane(a.c1r, a.c1i, x.v[1], x.v[0], true);
+              if (a_has_c1) {
+                av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], x.v[1], false);
+                av.v[1] = av.v[1] + zmul_lane(a_c1r, a_c1i, x.v[1], x.v[0], true);
               }
-              av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], s.v[1], false);
-              av.v[1] = av.v[1] + zmul_lane(a.c2r, a.c2i, s.v[1], s.v[0], true);
+              av.v[0] = av.v[0] + zmul_lane(a_c2r, a_c2i, s.v[0], s.v[1], false);
+              av.v[1] = av.v[1] + zmul_lane(a_c2r, a_c2i, s.v[1], s.v[0], true);
             } else {
               // the partner lane of a (re, im) pair is held by the neighbouring thread
               const bool odd = (l & 1) != 0;
-              if (a.has_c1) av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
-              av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
+              if (a_has_c1) av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
+              av.v[0] = av.v[0] + zmul_lane(a_c2r, a_c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
             }
-            stv<LPT>(a.acc + (size_t)q0 * L + l, av);
+            stv<LPT>(a_acc + (size_t)q0 * L + l, av);
           }
         }
       }
     }
   }
+}
+
+template <int LPT, int LC, int EPI, bool NL>
+__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
+  const int2 sc = za.sched[blockIdx.x];
+  if (sc.x < 0) return;              // padding entry of the schedule (whole workgroup)
+  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.x], sc.y);
+}
+
+// Several operators (systems) and batches in one launch: sched[b] = {descriptor, item,
+// lane chunk, -}; descriptors zs[] are ZArgs in device memory (their sched is unused).
+template <int LPT, int LC, int EPI, bool NL>
+__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch_multi(const ZArgs* __restrict__ zs,
+                                                                const int4* __restrict__ sched) {
+  const int4 sc = sched[blockIdx.x];
+  if (sc.x < 0) return;
+  const ZArgs& za = zs[sc.x];
+  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.y], sc.z);
 }
 
 // dynamic LDS bytes of k_zmarch for a lane chunk and segment length

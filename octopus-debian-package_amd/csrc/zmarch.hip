@@ -30,7 +30,42 @@ int launch_lc(const ZArgs& z, int epi, int grid, int zseg, hipStream_t s) {
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zmarch launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
+template <int LPT, int LC>
+int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool nl, int zseg, hipStream_t s) {
+  constexpr int NT = 64 * LC / LPT;
+  const size_t shmem = zmarch_shmem(LC, zseg);
+  ProfScope prof("k_apply", s);
+  if (epi == EPI_TAYLOR) {
+    if constexpr (LC >= 2) {
+      if (nl) hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_TAYLOR, true>), dim3(len), dim3(NT), shmem, s, zs, sched);
+      else hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_TAYLOR, false>), dim3(len), dim3(NT), shmem, s, zs, sched);
+    } else {
+      return fail(OCTHIP_ERR_INVALID, "z-march Taylor epilogue needs complex lane pairs");
+    }
+  } else {
+    if (nl) hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_STORE, true>), dim3(len), dim3(NT), shmem, s, zs, sched);
+    else hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_STORE, false>), dim3(len), dim3(NT), shmem, s, zs, sched);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zmarch_multi launch: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
 }  // namespace
+
+int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
+                        hipStream_t s) {
+  if (len <= 0) return OCTHIP_SUCCESS;
+  if (lpt == 2) {
+    switch (lc) {
+      case 2: return launch_multi_lc<2, 2>(zs, sched, len, epi, nl, zseg, s);
+      case 4: return launch_multi_lc<2, 4>(zs, sched, len, epi, nl, zseg, s);
+      case 8: return launch_multi_lc<2, 8>(zs, sched, len, epi, nl, zseg, s);
+      case 16: return launch_multi_lc<2, 16>(zs, sched, len, epi, nl, zseg, s);
+      default: break;
+    }
+  }
+  return fail(OCTHIP_ERR_INVALID, "no multi z-march instance for lane chunk %d, %d lanes per thread", lc, lpt);
+}
 
 int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, hipStream_t s) {
   if (lpt == 2) {

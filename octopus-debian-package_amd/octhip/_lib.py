@@ -54,6 +54,10 @@ SIGNATURES = {
     "octhip_adjoint_forward": [_P, _P, _P, _P],
     "octhip_adjoint_backward": [_P, _P, _P, _P, _P],
     "octhip_adjoint_destroy": [_P],
+    "octhip_batched_create": [_I, _P, _P, _P, _P, _D, _I, _P],
+    "octhip_batched_apply": [_P, _P],
+    "octhip_batched_taylor_step": [_P, _P],
+    "octhip_batched_destroy": [_P],
     "octhip_profiling_start": [],
     "octhip_profiling_stop": [],
     "octhip_profiling_get": [C.c_char_p, _DP, _IP],

@@ -281,3 +281,43 @@ class Adjoint:
             except Exception:  # interpreter shutdown: module globals may already be gone
                 pass
             self.handle = None
+
+
+class BatchedPropagator:
+    """Several (Hamiltonian, batch) pairs of uneven size propagated with one launch per
+    kernel kind and Taylor order (instead of the per-batch, per-system loops of
+    propagation_ops_elec_exp_apply, src/td/propagation_ops_elec.F90:232-249).  The
+    batches must stay alive, at the same addresses, for the life of the object."""
+
+    def __init__(self, pairs: list, deltat: float, order: int = 4, with_apply: bool = True):
+        """pairs: list of (HamiltonianDevice, Batch) with complex batches."""
+        if not pairs:
+            raise ValueError("BatchedPropagator needs at least one pair")
+        self.pairs = list(pairs)
+        self.hpsi = [Batch(b.nst, hm.np, True, b.pack_size) for hm, b in pairs] if with_apply else None
+        n = len(pairs)
+        hms = (C.c_void_p * n)(*[hm.handle for hm, _ in pairs])
+        psi = (C.c_void_p * n)(*[b.ptr() for _, b in pairs])
+        hp = (C.c_void_p * n)(*[h.ptr() for h in self.hpsi]) if with_apply else None
+        ldf = (C.c_int * n)(*[b.ldf for _, b in pairs])
+        for _, b in pairs:
+            if not b.is_complex:
+                raise ValueError("BatchedPropagator needs complex batches")
+        h = C.c_void_p()
+        call("octhip_batched_create", n, hms, psi, hp, ldf, float(deltat), int(order), C.byref(h))
+        self.handle = h
+
+    def apply(self) -> None:
+        """hpsi[i] = H_i psi[i] for every pair (results in self.hpsi)."""
+        call("octhip_batched_apply", self.handle, _stream())
+
+    def taylor_step(self) -> None:
+        call("octhip_batched_taylor_step", self.handle, _stream())
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _lib.lib().octhip_batched_destroy(self.handle)
+            except Exception:  # interpreter shutdown: module globals may already be gone
+                pass
+            self.handle = None

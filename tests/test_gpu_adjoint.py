@@ -113,6 +113,6 @@ def test_torch_autograd_harness(gpu):
     # psi0 untouched by the harness
     assert np.array_equal(psib.ff.cpu().numpy(), psi0)
     J2, psiN, chi0, deps, dv = device_run(gpu, s, vol, psi0, phi, eps, dt, 3)
-    assert float(J) == J2
+    assert float(J.detach()) == J2
     assert np.array_equal(e.grad.numpy(), 3.0 * deps)
     assert np.array_equal(v.grad.cpu().numpy(), 3.0 * dv)
