@@ -43,6 +43,21 @@ def tuning(args) -> dict:
     return t
 
 
+PROFILE = ROOT / "profiles" / "k_zmarch_hamiltonian_R12_h025_pack32.md"
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def hamiltonian_bytes(s, nst: int, complex_: bool = True) -> int:
     """Compulsory HBM bytes of one H-apply of a batch: psi read over np_part rows, H psi
     written over np rows, v, the projector matrices, maps and their points, weights."""
@@ -125,7 +140,9 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
                                "16 atoms x 4 KB projectors (rc=3)",
                    "states": nst_total, "pack_size": pack, "np": s.np, "np_part": s.np_part},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": args.traffic if args.traffic is not None else profiled_traffic(),
+                     "traffic_source": "--traffic" if args.traffic is not None else str(PROFILE.relative_to(ROOT)),
                      "kernel": "k_apply", "algorithmic_bytes_per_launch": bytes_batch,
                      "avg_launch_us": kern["k_apply"]["avg_us"]},
         "h_apply_hbm_frac": happly_gbs / HBM_PEAK_GBS,
