@@ -6,7 +6,7 @@ H = -1/2 Laplacian + v_loc + V_nl on the sphere R = 12 bohr, h = 0.25 (462,781 p
 order-4 star stencil, 64 states in two packed batches of 32, 16 Gaussian wells,
 16 atoms x 4 Kleinman-Bylander projectors.  One step = H applied to both batches.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor|adjoint]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor|adjoint|batched]
 
 Prints one JSON line.  `value` is state-points/s with the states already resident in
 HBM; `roofline` is the dominant kernel (k_apply, the fused stencil + local + ket pass)
@@ -21,6 +21,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "octopus-debian-package_amd"))
@@ -304,12 +306,82 @@ def bench_adjoint(args, torch, api, _lib, workloads):
             "k_adj_accum_avg_us": acc_ms * 1e3 / acc_n if acc_n else None}
 
 
+def bench_batched(args, torch, api, _lib, workloads):
+    """8 uneven spheres (R = 6..14, h = 0.25, 8..256 states): one H-apply + 50 Taylor-4
+    steps, as one batched launch sequence over all (system, batch) pairs and as plain
+    per-batch launches system by system; pack sizes chosen by the autotuner."""
+    log("building the 8 uneven systems")
+    systems = workloads.uneven_systems()
+    nsteps, dt = 50, args.dt
+    plan = []
+    for s, nst in systems:
+        hm = api.HamiltonianDevice.from_system(s)
+        best = None
+        for pack in (8, 16, 32, 64):
+            if pack > max(8, api.pad_pow2(nst)):
+                continue
+            ldf = int(np.log2(2 * pack))
+            us = hm.autotune(ldf, reps=3)
+            cost = -(-nst // pack) * us
+            if best is None or cost < best[0]:
+                best = (cost, pack, ldf, us)
+        hm.autotune(best[2], reps=3)  # leave the chosen shape configured
+        batches = []
+        for first in range(0, nst, best[1]):
+            n = min(best[1], nst - first)
+            z = workloads.gaussian_envelope_states(s, n, first=first)
+            batches.append(api.Batch.from_states(z, s.np_part, pack_size=best[1]))
+        plan.append((s, nst, hm, best[1], batches))
+        log(f"system np={s.np} nst={nst}: pack {best[1]} ({best[3]:.1f} us per batch H-apply)")
+    te = api.Exponential(4)
+
+    def run_plain():
+        for s, nst, hm, pack, batches in plan:
+            outs = [api.Batch(b.nst, s.np, True, b.pack_size) for b in batches]
+            for b, o in zip(batches, outs):
+                api.hamiltonian_elec_apply_batch(hm, b, o)
+            for _ in range(nsteps):
+                for b in batches:
+                    te.apply_batch(hm, b, dt)
+
+    pairs = [(hm, b) for s, nst, hm, pack, batches in plan for b in batches]
+    bp = api.BatchedPropagator(pairs, dt)
+
+    def run_batched():
+        bp.apply()
+        for _ in range(nsteps):
+            bp.taylor_step()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    timed(run_plain)  # warm-up (allocations of the per-handle work buffers)
+    timed(run_batched)
+    tp = min(timed(run_plain) for _ in range(max(1, min(args.steps, 3))))
+    tb = min(timed(run_batched) for _ in range(max(1, min(args.steps, 3))))
+    sp = sum(s.np * nst for s, nst, *_ in plan)
+    return {"metric": "batched vs plain launches (1 H-apply + 50 Taylor-4 steps, 8 uneven systems)",
+            "value": tp / tb, "unit": "speedup (plain time / batched time)", "higher_is_better": True,
+            "n_gpus": 1, "steps": nsteps, "warmup": 1, "ms_per_step": tb * 1e3 / nsteps, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "8 spheres R=6,8,9,10,11,12,13,14 h=0.25 with 8..256 states, 1 H-apply + "
+                                   "50 Taylor-4 steps", "packs": [p[3] for p in plan],
+                       "pairs": len(pairs)},
+            "plain_s": tp, "batched_s": tb,
+            "batched_state_point_happlies_per_s": sp * (1 + 4 * nsteps) / tb,
+            "plain_state_point_happlies_per_s": sp * (1 + 4 * nsteps) / tp}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor", "adjoint"])
+    ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor", "adjoint", "batched"])
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 gather, 2 z-march")
     ap.add_argument("--xcd-chunk", type=int, default=None)
     ap.add_argument("--gather-tile", type=int, default=None)
@@ -347,7 +419,7 @@ def main() -> int:
         log(f"calibration copy: {args.copy_calibration_mib} MiB read + written")
         del src, dst
     fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor,
-          "adjoint": bench_adjoint}[args.workload]
+          "adjoint": bench_adjoint, "batched": bench_batched}[args.workload]
     res = fn(args, torch, api, _lib, workloads)
     print(json.dumps(res), flush=True)
     return 0

@@ -256,6 +256,19 @@ int octhip_nl_operator_set_tuning(octhip_nl_operator_t op, int key, int value);
 int octhip_nl_operator_get_tuning(octhip_nl_operator_t op, int key, int* value);
 int octhip_hamiltonian_set_tuning(octhip_hamiltonian_t hm, int key, int value);
 
+/* Autotuner: times every candidate launch shape of the H-apply for rows of 2^ldf lanes
+ * (z-march lane chunk x lanes per thread x z segment, the tiled gather, MFMA or plain
+ * bra) on scratch batches (allocated here), reps applies each, on `stream`, and leaves
+ * the fastest on the handle; *best_us (optional) = its time per H-apply.  Results are
+ * kept in a process-wide table keyed by (np, np_part, number of projector matrices,
+ * ldf): unless force != 0 a known shape is configured without timing.  No counterpart in
+ * the reference (its device path has fixed launch shapes). */
+int octhip_hamiltonian_autotune(octhip_hamiltonian_t hm, int ldf, int reps, int force, void* stream,
+                                double* best_us);
+/* the tuning table as text (one line per shape) */
+int octhip_tuning_save(const char* path);
+int octhip_tuning_load(const char* path);
+
 #ifdef __cplusplus
 }
 #endif

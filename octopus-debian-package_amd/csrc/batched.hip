@@ -205,7 +205,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     OCTHIP_REQUIRE(ldf[i] >= 1 && ldf[i] <= MAX_LDF, "pair %d: ldf %d outside 1..%d (complex batches)", i, ldf[i],
                    MAX_LDF);
     const octhip_nl_operator_s* op = hms[i]->kin;
-    OCTHIP_REQUIRE(op->zmarch_ok && op->n == 25 && op->kernel != 1,
+    OCTHIP_REQUIRE(op->zmarch_ok && op->n == 25,
                    "pair %d: the batched launch needs the order-4 star on a lattice (z-march)", i);
     minL = std::min(minL, 1 << ldf[i]);
     const bool nl = hms[i]->nmat > 0 && hms[i]->nslot > 0;

@@ -19,9 +19,8 @@ enum { O_POINTS = 0, O_PROJS, O_MATRIX, O_MAP, O_SCAL, O_MIX };
 
 int ensure_work(octhip_hamiltonian_s* h, int ldf) {
   if (h->work_ldf >= ldf) return OCTHIP_SUCCESS;
-  dfree(h->d_partial);
-  dfree(h->d_proj);
-  dfree(h->d_ksum);
+  for (void* p : {(void*)h->d_partial, (void*)h->d_proj, (void*)h->d_ksum})
+    if (p) h->retired.push_back(p);
   h->d_partial = h->d_proj = h->d_ksum = nullptr;
   const size_t L = (size_t)1 << ldf;
   int rc = OCTHIP_SUCCESS;
@@ -38,7 +37,7 @@ int ensure_work(octhip_hamiltonian_s* h, int ldf) {
 int ensure_taylor(octhip_hamiltonian_s* h, int ldf) {
   if (h->y_ldf >= ldf) return OCTHIP_SUCCESS;
   for (auto& y : h->d_y) {
-    dfree(y);
+    if (y) h->retired.push_back(y);
     y = nullptr;
   }
   const size_t bytes = sizeof(double) * ((size_t)h->np << ldf);
@@ -283,6 +282,7 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_mat_chunk0);
   dfree(h->d_partial);
   dfree(h->d_proj);
+  for (void* p : h->retired) dfree(p);
   dfree(h->d_y[0]);
   dfree(h->d_y[1]);
   delete h;

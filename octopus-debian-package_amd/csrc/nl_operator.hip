@@ -171,9 +171,9 @@ int zmarch_items(octhip_nl_operator_s* op) {
         const int ze = std::min(zb + op->zseg, op->tile_zhi[t] + 1);
         items.push_back(make_int4(tx, ty, zb, ze));
       }
-  dfree(op->d_items);
+  if (op->d_items) op->retired.push_back(op->d_items);
   op->d_items = nullptr;
-  for (auto& e : op->sched) dfree(e.second);
+  for (auto& e : op->sched) op->retired.push_back(e.second);
   op->sched.clear();
   op->sched_len.clear();
   op->items = items;
@@ -335,6 +335,7 @@ extern "C" int octhip_nl_operator_destroy(octhip_nl_operator_t op) {
   dfree(op->d_mcube);
   dfree(op->d_items);
   for (auto& e : op->sched) dfree(e.second);
+  for (void* p : op->retired) dfree(p);
   dfree(op->d_ri);
   dfree(op->d_rimap);
   delete op;

@@ -115,6 +115,9 @@ struct octhip_nl_operator_s {
   std::vector<int4> items;
   std::vector<std::pair<int, int2*>> sched;  // lane chunks -> device schedule
   std::vector<int> sched_len;
+  // device buffers replaced by a rebuild stay allocated until destroy: captured graphs and
+  // batched launches may still point at them
+  std::vector<void*> retired;
 };
 
 struct octhip_hamiltonian_s {
@@ -159,6 +162,7 @@ struct octhip_hamiltonian_s {
   double* d_proj = nullptr;     // full_projection_size * L
   double* d_y[2] = {nullptr, nullptr};  // Taylor intermediates, np rows each
   int y_ldf = -1;
+  std::vector<void*> retired;   // work buffers outgrown by a larger ldf, freed at destroy
   int use_mfma = 1;
 };
 

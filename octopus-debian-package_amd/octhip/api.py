@@ -165,6 +165,13 @@ class HamiltonianDevice:
         for k, v in knobs.items():
             call("octhip_hamiltonian_set_tuning", self.handle, _lib.TUNE[k], int(v))
 
+    def autotune(self, ldf: int, reps: int = 5, force: bool = False) -> float:
+        """Pick the fastest launch shape for rows of 2^ldf lanes; returns us per H-apply."""
+        us = C.c_double()
+        call("octhip_hamiltonian_autotune", self.handle, int(ldf), int(reps), int(bool(force)), _stream(),
+             C.byref(us))
+        return us.value
+
     def set_potential(self, v: np.ndarray) -> None:
         v = np.ascontiguousarray(v, dtype=np.float64)
         call("octhip_hamiltonian_set_potential", self.handle, as_ptr(v, C.c_double), _stream())
@@ -321,3 +328,11 @@ class BatchedPropagator:
             except Exception:  # interpreter shutdown: module globals may already be gone
                 pass
             self.handle = None
+
+
+def tuning_save(path) -> None:
+    call("octhip_tuning_save", str(path).encode())
+
+
+def tuning_load(path) -> None:
+    call("octhip_tuning_load", str(path).encode())
