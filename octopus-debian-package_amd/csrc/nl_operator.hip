@@ -243,6 +243,9 @@ int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t
   int lpt = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
   if (lpt > lc) lpt = lc;
   if (lc == 32) lpt = 2;  // 1024 threads per workgroup at most
+  if (zmarch_shmem(lc, op->zseg_built) > 160 * 1024)
+    return fail(OCTHIP_ERR_INVALID, "z-march: lane chunk %d with z segment %d needs more than 160 KB of LDS", lc,
+                op->zseg_built);
   const int2* sched = nullptr;
   int len = 0;
   rc = zmarch_sched(op, a.L / lc, &sched, &len);

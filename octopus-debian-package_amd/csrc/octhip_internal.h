@@ -104,7 +104,7 @@ struct octhip_nl_operator_s {
   bool zmarch_ok = false;
   int* d_mcube = nullptr;
   int mx = 0, my = 0, mz = 0;
-  int zseg = 16;                         // output planes per work item
+  int zseg = 32;                         // output planes per work item (sweep on MI355X: 32 best)
   int lane_chunk = 0;                    // lanes per workgroup (0 = automatic)
   int lanes_per_thread = 0;              // doubles per thread (0 = automatic)
   int zseg_built = 0;

@@ -55,6 +55,8 @@ int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool n
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
                         hipStream_t s) {
   if (len <= 0) return OCTHIP_SUCCESS;
+  if (zmarch_shmem(lc, zseg) > 160 * 1024)
+    return fail(OCTHIP_ERR_INVALID, "z-march: lane chunk %d with z segment %d needs more than 160 KB of LDS", lc, zseg);
   if (lpt == 2) {
     switch (lc) {
       case 2: return launch_multi_lc<2, 2>(zs, sched, len, epi, nl, zseg, s);
