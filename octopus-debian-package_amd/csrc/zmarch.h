@@ -26,6 +26,12 @@ constexpr int ZR = 4;                // stencil reach
 constexpr int ZW = ZT + 2 * ZR;      // window side (tile + halo)
 constexpr int ZS = 192;              // sites of the cross-shaped plane window: 64 tile + 128 halo
 constexpr int ZPADZ = 8;             // march-cube padding below/above the inner points in z
+#ifndef ZM_HALO_AHEAD
+#define ZM_HALO_AHEAD 2                // planes of halo values loaded ahead (deeper measured slower)
+#endif
+#ifndef ZM_AUX_AHEAD
+#define ZM_AUX_AHEAD 3                 // planes of output-point data loaded ahead
+#endif
 
 struct ZArgs {
   ApplyArgs a;
@@ -193,37 +199,44 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   // Rings with static slots (the z loop is unrolled by RQ, so every slot index is a
   // compile-time constant and values stay in the registers their loads wrote):
   //   ov[(P - zb + 4) % RQ]  own value of plane P,  P in z-4 .. z+6 (loaded 6 planes ahead)
-  //   hv[(P - zb) % 4]       halo values of plane P, P in z .. z+2  (loaded 2 planes ahead)
-  //   vr/er/ar[(P - zb) % 4] output point data,      P in z .. z+3  (loaded 3 planes ahead)
+  //   hv[(P - zb) % HR]      halo values of plane P, P in z .. z+HD (loaded HD planes ahead)
+  //   vr/kr/ar[(P - zb) % AR] output point data,     P in z .. z+AD (loaded AD planes ahead)
+  // The load counter retires in order, so the shortest distance sets how many planes of
+  // loads stay in flight: halo and point data go as deep as the own values allow.
   constexpr int RQ = 12;
+  constexpr int HD = ZM_HALO_AHEAD, AD = ZM_AUX_AHEAD;
+  constexpr int HR = HD < 3 ? HD + 1 + (HD == 1) : 6;  // ring sizes dividing RQ
+  constexpr int AR = AD < 3 ? AD + 1 + (AD == 1) : 6;
+  static_assert(RQ % HR == 0 && RQ % AR == 0 && HR > HD && AR > AD && HD <= 6 && AD <= 6, "ring sizes");
   Vec<LPT> ov[RQ];
-  Vec<LPT> hv[4][2];
-  double vr[4] = {0.0, 0.0, 0.0, 0.0};
-  Vec<LPT> kr[4];
-  Vec<LPT> ar[4];
+  Vec<LPT> hv[HR][2];
+  double vr[AR] = {};
+  Vec<LPT> kr[AR];
+  Vec<LPT> ar[AR];
   const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
-  // per-point data of plane P's output point: potential, ket sum row, accumulator row
+  // Every global load below is issued unconditionally (out-of-range requests read the
+  // zero row): with loads behind branches the compiler cannot count the loads still in
+  // flight and drains the whole prefetch queue at every plane.
+  // per-point data of plane P (< ze + 4) for its output point: potential, ket sum row,
+  // accumulator row
   auto load_aux = [&](int slot, int P) {
     const int q = own_idx(P);
-    if ((unsigned)q < (unsigned)a_np) {
-      if (a_v) vr[slot] = a_v[q];
-      if constexpr (NL) {
-        const int ks = tks[(P - zb + 4) * 64 + col];
-        const double* src = ks >= 0 ? a_ksum + (size_t)ks * L + l : a_zrow;
-        kr[slot] = ldv<LPT>(src);
-      }
-      if (has_acc) ar[slot] = ldv<LPT>(a_acc + (size_t)q * L + l);
+    const bool ok = P < ze && (unsigned)q < (unsigned)a_np;
+    vr[slot] = *((a_v && ok) ? a_v + q : a_zrow);
+    if constexpr (NL) {
+      const int ks = tks[(P - zb + 4) * 64 + col];
+      kr[slot] = ldv<LPT>((ok && ks >= 0) ? a_ksum + (size_t)ks * L + l : a_zrow);
     }
+    if constexpr (EPI == EPI_TAYLOR) ar[slot] = ldv<LPT>((has_acc && ok) ? a_acc + (size_t)q * L + l : a_zrow);
   };
 #pragma unroll
   for (int k = 0; k < 10; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
 #pragma unroll
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < HD; ++k)
 #pragma unroll
-    for (int i = 0; i < 2; ++i) hv[k][i] = (zb + k < ze) ? val(halo_idx(zb + k, i)) : ov[0];
+    for (int i = 0; i < 2; ++i) hv[k][i] = val(zb + k < ze ? halo_idx(zb + k, i) : -1);
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
-    if (zb + k < ze) load_aux(k, zb + k);
+  for (int k = 0; k < AD; ++k) load_aux(k, zb + k);
 
   for (int z0 = zb; z0 < ze; z0 += RQ) {
 #pragma unroll
@@ -231,17 +244,15 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       const int z = z0 + kk;
       if (z >= ze) break;
       const int buf = kk & 1;
-      // prefetch: output-point data of plane z+3, own column z+6, halo of plane z+2
-      if (z + 3 < ze) load_aux((kk + 3) & 3, z + 3);
-      if (z + 6 < ze + 4) ov[(kk + 10) % RQ] = val(own_idx(z + 6));
-      if (z + 2 < ze) {
+      // prefetch: output-point data of plane z+AD, own column z+6, halo of plane z+HD
+      load_aux((kk + AD) % AR, z + AD);
+      ov[(kk + 10) % RQ] = val(z + 6 < ze + 4 ? own_idx(z + 6) : -1);
 #pragma unroll
-        for (int i = 0; i < 2; ++i) hv[(kk + 2) & 3][i] = val(halo_idx(z + 2, i));
-      }
+      for (int i = 0; i < 2; ++i) hv[(kk + HD) % HR][i] = val(z + HD < ze ? halo_idx(z + HD, i) : -1);
       // stage plane z: tile centre and halo strips
       lds_st(buf, own_lds, ov[(kk + 4) % RQ]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) lds_st(buf, hlds[i], hv[kk & 3][i]);
+      for (int i = 0; i < 2; ++i) lds_st(buf, hlds[i], hv[kk % HR][i]);
       __syncthreads();
       // output point (x0+cx, y0+cy, z)
       const int q0 = own_idx(z);
@@ -267,20 +278,20 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
         }
         const Vec<LPT> x = ov[(kk + 4) % RQ];
         if (a_v) {
-          const double vv = vr[kk & 3];
+          const double vv = vr[kk % AR];
 #pragma unroll
           for (int k = 0; k < LPT; ++k) s.v[k] = fma(vv, x.v[k], s.v[k]);
         }
         if constexpr (NL) {
 #pragma unroll
-          for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kr[kk & 3].v[k];
+          for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kr[kk % AR].v[k];
         }
         if constexpr (EPI == EPI_STORE) {
           stv<LPT>(a_fo + (size_t)q0 * L + l, s);
         } else {
           if (a_yout) stv<LPT>(a_yout + (size_t)q0 * L + l, s);
           if (has_acc) {
-            Vec<LPT> av = ar[kk & 3];
+            Vec<LPT> av = ar[kk % AR];
             if constexpr (LPT == 2) {
               if (a_has_c1) {
                 av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], x.v[1], false);
