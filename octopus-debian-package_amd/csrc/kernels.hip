@@ -113,7 +113,11 @@ __global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) { bra_plai
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 
-__device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c) {
+// One workgroup per (chunk, lane tile of 16 lanes): grids of nchunk * L/16 blocks keep
+// every CU busy; the 4 waves take interleaved groups of 4 sphere points and are summed
+// through LDS in a fixed order.  Every load is issued unconditionally (clamped indices,
+// zero operands past the chunk end) so the compiler counts them exactly.
+__device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int lt) {
   constexpr int KB = 16;                    // MFMA steps (4 sphere points each) per load block
   __shared__ double red[4 * 16 * 16];
   const BraArgs& b = m.b;
@@ -123,64 +127,50 @@ __device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c) {
   const int wave = threadIdx.x >> 6;
   const int ln = threadIdx.x & 63;
   const int kk = ln >> 4, ii = ln & 15;
-  const int nlt = b.L >> 4;                 // lane tiles of 16
-  // waves split lane tiles; if fewer tiles than waves, they also split the sphere points
-  const int nsl = nlt >= 4 ? 1 : 4 / nlt;   // slices over points
-  const int my_sl = nlt >= 4 ? 0 : wave / nlt;
-  const int t0 = nlt >= 4 ? wave : wave % nlt;
-  const int tstep = nlt >= 4 ? 4 : nlt;
-  // operand loads of one MFMA step: lane (kk, ii) supplies A[ii][kk] and B[kk][ii]
-  auto load_ab = [&](int q, int jb, int col, double& av, double& bv) {
-    const int qk = q + kk;
-    av = 0.0;
-    bv = 0.0;
-    if (qk < ch.z) {
-      if (jb + ii < npj) av = m.pt[(size_t)(mapoff + qk) * m.pjs + jb + ii];
-      const int row = b.maps[mapoff + qk] - 1;
-      bv = b.psi[(size_t)row * b.L + col];
-    }
-  };
+  const int col = lt * 16 + ii;
   for (int jb = 0; jb < npj; jb += 16) {
-    for (int lt = t0; lt < nlt; lt += tstep) {
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
-      const int col = lt * 16 + ii;
-      const int qstep = 4 * nsl;
-      // blocks of KB steps; each block's loads are issued together before its MFMAs
-      for (int q0 = ch.y + 4 * my_sl; q0 < ch.z; q0 += KB * qstep) {
-        double av[KB], bv[KB];
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int jc = min(jb + ii, npj - 1);
+    const bool jok = jb + ii < npj;
+    for (int q0 = ch.y + 4 * wave; q0 < ch.z; q0 += KB * 16) {
+      int row[KB];
+      double av[KB], bv[KB];
 #pragma unroll
-        for (int k = 0; k < KB; ++k) load_ab(q0 + k * qstep, jb, col, av[k], bv[k]);
-#pragma unroll
-        for (int k = 0; k < KB; ++k) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], bv[k], acc, 0, 0, 0);
+      for (int k = 0; k < KB; ++k) {
+        const int qk = q0 + 16 * k + kk;
+        const int qs = qk < ch.z ? qk : ch.y;
+        row[k] = b.maps[mapoff + qs] - 1;
+        av[k] = m.pt[(size_t)(mapoff + qs) * m.pjs + jc];
       }
-      if (nsl == 1) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int j = jb + 4 * r + kk;
-          if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = acc[r];
-        }
-      } else {
-        // deterministic slice reduction through LDS
-        __syncthreads();
+      for (int k = 0; k < KB; ++k) bv[k] = b.psi[(size_t)row[k] * b.L + col];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[(wave * 16 + 4 * r + kk) * 16 + ii] = acc[r];
-        __syncthreads();
-        if (my_sl == 0) {
+      for (int k = 0; k < KB; ++k) {
+        const bool ok = q0 + 16 * k + kk < ch.z;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((ok && jok) ? av[k] : 0.0, ok ? bv[k] : 0.0, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int jl = 4 * r + kk;
-            double t = 0.0;
-            for (int s2 = 0; s2 < nsl; ++s2) t += red[((s2 * nlt + t0) * 16 + jl) * 16 + ii];
-            const int j = jb + jl;
-            if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = t;
-          }
-        }
+    for (int r = 0; r < 4; ++r) red[(wave * 16 + 4 * r + kk) * 16 + ii] = acc[r];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int jl = 4 * r + kk;
+        const double t = (red[(0 * 16 + jl) * 16 + ii] + red[(1 * 16 + jl) * 16 + ii]) +
+                         (red[(2 * 16 + jl) * 16 + ii] + red[(3 * 16 + jl) * 16 + ii]);
+        const int j = jb + jl;
+        if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = t;
       }
     }
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) { bra_mfma_body(m, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
+  const int nlt = m.b.L >> 4;
+  bra_mfma_body(m, blockIdx.x / nlt, blockIdx.x % nlt);
+}
 
 // proj[i, l] = scal[i] * sum_{chunks of matrix(i)} partial[chunk, i - pbase, l]
 __device__ __forceinline__ void bra_reduce_body(const BraReduceArgs& r, int bid, int nblk) {
@@ -226,7 +216,7 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
   if (nchunk <= 0) return OCTHIP_SUCCESS;
   BraMfmaArgs m{b, pt, pjs};
   ProfScope prof("k_bra", s);
-  hipLaunchKernelGGL(k_bra_mfma, dim3(nchunk), dim3(BLOCK), 0, s, m);
+  hipLaunchKernelGGL(k_bra_mfma, dim3(nchunk * (b.L >> 4)), dim3(BLOCK), 0, s, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra_mfma launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
@@ -236,35 +226,41 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
 // D(16 points x 16 lanes) = A(16 points x 4 projectors) B(4 projectors x 16 lanes); lane
 // (kk, ii) supplies A[ii][kk] = P[q0+ii, jb+kk] and B[kk][ii] = proj[pbase+jb+kk, col];
 // accumulator element r of lane (kk, ii) is D[4r+kk][ii].
-__device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c) {
+// One workgroup per (chunk, lane tile of 16); the 4 waves take the chunk's groups of 16
+// sphere points in turn.  Loads are unconditional (clamped indices, zero operands).
+__device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
   const int4 ch = k.chunks[c];
   const int4 mi = k.mat_info[ch.x];
   const int npj = mi.y, mapoff = mi.w;
   const int pb = k.mat_pbase[ch.x];
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int kk = ln >> 4, ii = ln & 15;
-  const int nlt = k.L >> 4;
-  const int npt = ch.z - ch.y;
-  const int ntile = ((npt + 15) >> 4) * nlt;
-  for (int tl = wave; tl < ntile; tl += 4) {
-    const int pt0 = ch.y + (tl / nlt) * 16;
-    const int col = (tl % nlt) * 16 + ii;
+  const int col = lt * 16 + ii;
+  const int ngrp = (ch.z - ch.y + 15) >> 4;
+  for (int g = wave; g < ngrp; g += 4) {
+    const int pt0 = ch.y + g * 16;
+    const int q = pt0 + ii;
+    const int qs = q < ch.z ? q : ch.y;
     d4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int jb = 0; jb < npj; jb += 4) {
-      const int q = pt0 + ii;
-      const double av = (q < ch.z && jb + kk < npj) ? k.pt[(size_t)(mapoff + q) * k.pjs + jb + kk] : 0.0;
-      const double bv = (jb + kk < npj) ? k.proj[(size_t)(pb + jb + kk) * k.L + col] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      const bool jok = jb + kk < npj;
+      const int jj = jok ? jb + kk : npj - 1;
+      const double a = k.pt[(size_t)(mapoff + qs) * k.pjs + jj];
+      const double bb = k.proj[(size_t)(pb + jj) * k.L + col];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((q < ch.z && jok) ? a : 0.0, jok ? bb : 0.0, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int q = pt0 + 4 * r + kk;
-      if (q < ch.z) k.kbuf[(size_t)(mapoff + q) * k.L + col] = acc[r];
+      const int qo = pt0 + 4 * r + kk;
+      if (qo < ch.z) k.kbuf[(size_t)(mapoff + qo) * k.L + col] = acc[r];
     }
   }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) { ket_mfma_body(k, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) {
+  const int nlt = k.L >> 4;
+  ket_mfma_body(k, blockIdx.x / nlt, blockIdx.x % nlt);
+}
 
 // plain form, any row length: one thread per (sphere point, lane)
 __device__ __forceinline__ void ket_plain_body(const KetArgs& k, int c) {
@@ -301,7 +297,7 @@ int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
   {
     ProfScope prof("k_ket", s);
     if (mfma && k.L >= 16)
-      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk), dim3(BLOCK), 0, s, k);
+      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk * (k.L >> 4)), dim3(BLOCK), 0, s, k);
     else
       hipLaunchKernelGGL(k_ket_plain, dim3(nchunk), dim3(BLOCK), 0, s, k);
     hipError_t e = hipGetLastError();
@@ -351,7 +347,8 @@ __device__ __forceinline__ int segment_of(const int* __restrict__ off, int nseg,
 __global__ __launch_bounds__(BLOCK) void k_bra_mfma_multi(const BraMfmaArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
-  bra_mfma_body(a[i], blockIdx.x - off[i]);
+  const int b = blockIdx.x - off[i], nlt = a[i].b.L >> 4;
+  bra_mfma_body(a[i], b / nlt, b % nlt);
 }
 __global__ __launch_bounds__(BLOCK) void k_bra_plain_multi(const BraMfmaArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
@@ -366,7 +363,8 @@ __global__ __launch_bounds__(BLOCK) void k_bra_reduce_multi(const BraReduceArgs*
 __global__ __launch_bounds__(BLOCK) void k_ket_mfma_multi(const KetArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
-  ket_mfma_body(a[i], blockIdx.x - off[i]);
+  const int b = blockIdx.x - off[i], nlt = a[i].L >> 4;
+  ket_mfma_body(a[i], b / nlt, b % nlt);
 }
 __global__ __launch_bounds__(BLOCK) void k_ket_plain_multi(const KetArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
