@@ -8,7 +8,8 @@ using namespace octhip;
 
 namespace octhip {
 void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a);
-int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s);
+int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s, double* zf = nullptr,
+                    size_t zfirst = 0, size_t zcount = 0);
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s);
 int prepare_operator(octhip_nl_operator_s* op, int L);
 }  // namespace octhip
@@ -49,9 +50,19 @@ int ensure_taylor(octhip_hamiltonian_s* h, int ldf) {
   return OCTHIP_SUCCESS;
 }
 
-// projections of batch f (L lanes) into h->d_proj
-int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s) {
-  if (h->nmat == 0) return OCTHIP_SUCCESS;
+int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s);
+
+// projections of batch f (L lanes) into h->d_proj; zero_bc (optional) = a batch whose
+// boundary rows are cleared on the way (fused into the bra launch when it can be)
+int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s, double* zero_bc = nullptr) {
+  const int np = h->kin->np, np_part = h->kin->np_part;
+  if (h->nmat == 0) return zero_bc ? zero_boundary(zero_bc, np, np_part, L, s) : OCTHIP_SUCCESS;
+  const bool mfma = h->use_mfma && L >= 16;
+  if (zero_bc && !mfma) {
+    int rc = zero_boundary(zero_bc, np, np_part, L, s);
+    if (rc) return rc;
+    zero_bc = nullptr;
+  }
   BraArgs b;
   b.psi = f;
   b.L = L;
@@ -61,8 +72,9 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s) {
   b.maps = h->d_maps;
   b.partial = h->d_partial;
   b.pjmax = h->max_nprojs;
-  int rc = (h->use_mfma && L >= 16) ? launch_bra_mfma(b, h->d_pt, h->pjs, h->nchunk, s)
-                                    : launch_bra(b, h->nchunk, false, s);
+  int rc = mfma ? launch_bra_mfma(b, h->d_pt, h->pjs, h->nchunk, s, zero_bc, (size_t)np * L,
+                                  (size_t)(np_part - np) * L)
+                : launch_bra(b, h->nchunk, false, s);
   if (rc) return rc;
   BraReduceArgs r;
   r.partial = h->d_partial;
@@ -342,12 +354,12 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   const octhip_nl_operator_s* op = h->kin;
   int rc = ensure_work(h, ldf);
   if (rc) return rc;
-  if (set_bc) {
-    rc = zero_boundary(psi, op->np, op->np_part, L, s);
-    if (rc) return rc;
-  }
+  // boundaries_set: the boundary rows are cleared inside the bra launch when there is one
   if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
-    rc = run_bra(h, psi, L, s);
+    rc = run_bra(h, psi, L, s, set_bc ? psi : nullptr);
+    if (rc) return rc;
+  } else if (set_bc) {
+    rc = zero_boundary(psi, op->np, op->np_part, L, s);
     if (rc) return rc;
   }
   ApplyArgs a;
@@ -491,8 +503,13 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
                 double* const* save, hipStream_t s) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
-  int rc = zero_boundary(psi, op->np, op->np_part, L, s);
-  if (rc) return rc;
+  int rc = OCTHIP_SUCCESS;
+  // psi's boundary rows are cleared by the first bra launch (or here when there is no
+  // nonlocal term); every pass reads boundary rows as zero (lim = np) in any case
+  if (h->nmat == 0) {
+    rc = zero_boundary(psi, op->np, op->np_part, L, s);
+    if (rc) return rc;
+  }
   if (save) OCTHIP_HIP(hipMemcpyAsync(save[0], psi, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
   std::complex<double> zf[33];
   taylor_coefs(deltat, imag_time, order, zf);
@@ -501,7 +518,7 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
     z = zf[k];
     const double* yin = (k == 1) ? psi : (save ? save[k - 1] : h->d_y[(k - 2) & 1]);
     double* yout = (k < order) ? (save ? save[k] : h->d_y[(k - 1) & 1]) : nullptr;
-    rc = run_bra(h, yin, L, s);
+    rc = run_bra(h, yin, L, s, (k == 1 && h->nmat > 0) ? psi : nullptr);
     if (rc) return rc;
     ApplyArgs a;
     std::memset(&a, 0, sizeof(a));

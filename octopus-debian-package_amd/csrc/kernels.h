@@ -278,6 +278,10 @@ struct BraMfmaArgs {
   BraArgs b;
   const double* __restrict__ pt;  // total_points x pjs
   int pjs;
+  // optional: rows [zfirst, zfirst + zcount) of zf cleared by blocks nbra.. of the launch
+  double* zf;
+  size_t zfirst, zcount;
+  int nbra;                       // blocks doing the bra: chunks x 16-lane tiles
 };
 
 struct ZeroArgs {

@@ -168,6 +168,14 @@ __device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int l
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
+  if ((int)blockIdx.x >= m.nbra) {
+    // blocks past the bra clear the boundary rows of the batch (boundaries_set), fused
+    // into this launch: the bra reads inner rows only
+    const size_t nb = gridDim.x - m.nbra;
+    for (size_t t = (blockIdx.x - m.nbra) * (size_t)BLOCK + threadIdx.x; t < m.zcount; t += nb * BLOCK)
+      m.zf[m.zfirst + t] = 0.0;
+    return;
+  }
   const int nlt = m.b.L >> 4;
   bra_mfma_body(m, blockIdx.x / nlt, blockIdx.x % nlt);
 }
@@ -212,11 +220,13 @@ int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s) {
   return OCTHIP_SUCCESS;
 }
 
-int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s) {
+int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s, double* zf,
+                    size_t zfirst, size_t zcount) {
   if (nchunk <= 0) return OCTHIP_SUCCESS;
-  BraMfmaArgs m{b, pt, pjs};
+  BraMfmaArgs m{b, pt, pjs, zf, zfirst, zf ? zcount : 0, nchunk * (b.L >> 4)};
+  const int zblocks = zf ? (int)std::min<size_t>((zcount + BLOCK - 1) / BLOCK, 2048) : 0;
   ProfScope prof("k_bra", s);
-  hipLaunchKernelGGL(k_bra_mfma, dim3(nchunk * (b.L >> 4)), dim3(BLOCK), 0, s, m);
+  hipLaunchKernelGGL(k_bra_mfma, dim3(m.nbra + zblocks), dim3(BLOCK), 0, s, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_bra_mfma launch: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
@@ -283,8 +293,17 @@ __device__ __forceinline__ void ket_sum_body(const KetArgs& k, int bid, int nblk
   for (size_t t = bid * (size_t)BLOCK + threadIdx.x; t < total; t += (size_t)nblk * BLOCK) {
     const int m = (int)(t / k.L);
     const int l = (int)(t - (size_t)m * k.L);
+    const int e0 = k.multi_pos[m], e1 = k.multi_pos[m + 1];
     double s = 0.0;
-    for (int e = k.multi_pos[m]; e < k.multi_pos[m + 1]; ++e) s = s + k.kbuf[(size_t)k.multi_g[e] * k.L + l];
+    // rows in groups of 4 loaded together (clamped indices), added in entry order
+    for (int e = e0; e < e1; e += 4) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = k.kbuf[(size_t)k.multi_g[min(e + u, e1 - 1)] * k.L + l];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e + u < e1) s = s + v[u];
+    }
     k.kbuf[((size_t)k.total_points + m) * k.L + l] = s;
   }
 }
