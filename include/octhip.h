@@ -230,6 +230,9 @@ int octhip_adjoint_destroy(octhip_adjoint_t adj);
 int octhip_profiling_start(void);
 int octhip_profiling_stop(void);
 int octhip_profiling_get(const char* region, double* total_ms, int* count);
+/* record only one region (each record is two HIP events on the launch stream, which
+ * costs device time between kernels); NULL or "" records every region again */
+int octhip_profiling_select(const char* region);
 
 /* ------------------------------------------------------------------ tuning --
  * No counterpart in the reference (Octopus fixes launch shapes in accel.F90).  Key/value

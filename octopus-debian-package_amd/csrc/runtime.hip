@@ -50,6 +50,7 @@ struct ProfRec {
 };
 bool g_prof = false;
 bool g_capturing = false;
+char g_only[64] = "";           // record only this region when set
 std::vector<ProfRec> g_recs;
 }  // namespace
 
@@ -57,6 +58,7 @@ void set_capturing(bool on) { g_capturing = on; }
 
 ProfScope::ProfScope(const char* region, hipStream_t s) : s_(s) {
   if (!g_prof || g_capturing) return;
+  if (g_only[0] && std::strcmp(g_only, region) != 0) return;
   ProfRec r{region, nullptr, nullptr};
   if (hipEventCreate(&r.start) != hipSuccess || hipEventCreate(&r.stop) != hipSuccess) return;
   if (hipEventRecord(r.start, s) != hipSuccess) return;
@@ -84,6 +86,16 @@ extern "C" int octhip_profiling_start(void) {
   OCTHIP_REQUIRE_INIT();
   prof_clear();
   g_prof = true;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_profiling_select(const char* region) {
+  if (!region || !region[0]) {
+    g_only[0] = '\0';
+    return OCTHIP_SUCCESS;
+  }
+  OCTHIP_REQUIRE(std::strlen(region) < sizeof(g_only), "region name too long");
+  std::strcpy(g_only, region);
   return OCTHIP_SUCCESS;
 }
 

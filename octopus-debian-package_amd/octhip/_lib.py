@@ -61,6 +61,7 @@ SIGNATURES = {
     "octhip_hamiltonian_autotune": [_P, _I, _I, _I, _P, _P],
     "octhip_tuning_save": [C.c_char_p],
     "octhip_tuning_load": [C.c_char_p],
+    "octhip_profiling_select": [C.c_char_p],
     "octhip_profiling_start": [],
     "octhip_profiling_stop": [],
     "octhip_profiling_get": [C.c_char_p, _DP, _IP],
@@ -134,6 +135,11 @@ def as_ptr(a, ctype):
 
 def profiling_start() -> None:
     call("octhip_profiling_start")
+
+
+def profiling_select(region: str | None) -> None:
+    """Record only `region` (None: every region)."""
+    call("octhip_profiling_select", (region or "").encode())
 
 
 def profiling_stop() -> None:
