@@ -195,6 +195,9 @@ int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms, double* c
 int octhip_batched_apply(octhip_batched_t batched, void* stream);
 /* psi[i] <- sum_k (-i deltat H_i)^k / k! psi[i] for every pair */
 int octhip_batched_taylor_step(octhip_batched_t batched, void* stream);
+/* one batched Taylor step captured as a graph (launch and destroy with the
+ * octhip_taylor_graph_* entry points) */
+int octhip_batched_graph_create(octhip_batched_t batched, void* stream, octhip_taylor_graph_t* graph);
 int octhip_batched_destroy(octhip_batched_t batched);
 
 /* ---------------------------------------------------------------- adjoint --

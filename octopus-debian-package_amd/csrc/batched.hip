@@ -167,7 +167,8 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
     z.items = op->d_items;
     z.sched = nullptr;
     // every pointer the kernel dereferences must exist
-    const bool ok = z.a.fi && z.a.zrow && z.mcube && z.items && (k == 0 ? z.a.fo != nullptr : true) &&
+    // (the H-apply descriptor may lack fo: octhip_batched_apply refuses to run without hpsi)
+    const bool ok = z.a.fi && z.a.zrow && z.mcube && z.items &&
                     (!z.a.kslot || z.a.ksum) && (k < 2 || z.a.acc) && (k == 0 || k == b->order || z.a.yout);
     if (!ok) return fail(OCTHIP_ERR_STATE, "batched: incomplete descriptor for pair %zu, pass %d", i, k);
   }
@@ -354,6 +355,13 @@ extern "C" int octhip_batched_taylor_step(octhip_batched_t b, void* stream) {
   int rc = launch_multi(MULTI_ZERO, b->zero.args, b->zero.off, b->zero.nseg, b->zero.nblocks, s);
   for (int k = 1; k <= b->order && !rc; ++k) rc = run_pass(b, k, EPI_TAYLOR, s);
   return rc;
+}
+
+extern "C" int octhip_batched_graph_create(octhip_batched_t b, void* stream, octhip_taylor_graph_t* out) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(b && out, "null argument");
+  *out = nullptr;
+  return capture_graph((hipStream_t)stream, [&](hipStream_t cs) { return octhip_batched_taylor_step(b, cs); }, out);
 }
 
 extern "C" int octhip_batched_destroy(octhip_batched_t b) {

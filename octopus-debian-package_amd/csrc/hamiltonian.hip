@@ -1,5 +1,6 @@
 // Hamiltonian handle: local potential, KB projectors, fused apply, Taylor propagator.
 #include <complex>
+#include <functional>
 #include <cstring>
 
 #include "kernels.h"
@@ -443,7 +444,19 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
   *out = nullptr;
   int rc = prepare_taylor(h, ldf);
   if (rc) return rc;
-  hipStream_t s = (hipStream_t)stream;
+  return capture_graph((hipStream_t)stream,
+                       [&](hipStream_t cs) {
+                         int r = OCTHIP_SUCCESS;
+                         for (int b = 0; b < nbatch && !r; ++b)
+                           r = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs);
+                         return r;
+                       },
+                       out);
+}
+
+namespace octhip {
+// Capture the launches `body` issues on a capturing stream into an instantiated graph.
+int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out) {
   hipStream_t cs = s;
   bool own = false;
   if (!cs) {  // the legacy default stream cannot be captured
@@ -457,7 +470,7 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
     return fail(OCTHIP_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
   }
   set_capturing(true);
-  for (int b = 0; b < nbatch && !rc; ++b) rc = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs);
+  int rc = body(cs);
   set_capturing(false);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(cs, &graph);
@@ -478,6 +491,7 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
   *out = g;
   return OCTHIP_SUCCESS;
 }
+}  // namespace octhip
 
 extern "C" int octhip_taylor_graph_launch(octhip_taylor_graph_t g, void* stream) {
   OCTHIP_REQUIRE_INIT();

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <complex>
+#include <functional>
 
 #include <hip/hip_runtime.h>
 
@@ -172,6 +173,8 @@ int prepare_taylor(octhip_hamiltonian_s* h, int ldf);
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
                 double* const* save, hipStream_t s);
 void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z);
+// launches of body(capturing stream) as one instantiated graph (octhip_taylor_graph_launch)
+int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
 int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
 }  // namespace octhip

@@ -321,6 +321,14 @@ class BatchedPropagator:
     def taylor_step(self) -> None:
         call("octhip_batched_taylor_step", self.handle, _stream())
 
+    def graph(self) -> "TaylorGraph":
+        """One batched Taylor step captured as a HIP graph (TaylorGraph.launch())."""
+        h = C.c_void_p()
+        call("octhip_batched_graph_create", self.handle, _stream(), C.byref(h))
+        g = TaylorGraph.__new__(TaylorGraph)
+        g.hm, g.batches, g.owner, g.handle = None, [b for _, b in self.pairs], self, h
+        return g
+
     def __del__(self):
         if getattr(self, "handle", None):
             try:

@@ -75,3 +75,21 @@ def test_batched_rejects_mixed_projector_sets(gpu):
         p.append((hm, api.Batch.from_states(workloads.gaussian_envelope_states(s, 4), s.np_part)))
     with pytest.raises(api._lib.OcthipError):
         api.BatchedPropagator(p, 0.01)
+
+
+def test_batched_graph_replays_bitwise(gpu):
+    print("[batched graph]", flush=True)
+    pairs, _ = make_pairs(systems())
+    twin = [(hm, b.clone()) for hm, b in pairs]
+    bp = api.BatchedPropagator(pairs, 0.01, with_apply=False)
+    bq = api.BatchedPropagator(twin, 0.01, with_apply=False)
+    g = bq.graph()
+    other = twin[0][1].clone()
+    hother = api.Batch(other.nst, twin[0][0].np, True, other.pack_size)
+    for rep in range(3):
+        bp.taylor_step()
+        g.launch()
+        api.hamiltonian_elec_apply_batch(twin[0][0], other, hother)  # unrelated work between replays
+        gpu.cuda.synchronize()
+        for (_, a), (_, b) in zip(pairs, twin):
+            assert gpu.equal(a.ff[: a.nrows], b.ff[: b.nrows]), f"replay {rep}"
