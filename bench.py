@@ -307,6 +307,10 @@ def bench_adjoint(args, torch, api, _lib, workloads):
     t_f = min(x[0] for x in f)
     t_b = min(x[1] for x in f)
     acc_ms, acc_n = _lib.profiling_get("k_adj_accum")
+    breakdown = {}
+    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_adj_accum", "k_zero_rows"):
+        t, n = _lib.profiling_get(k)
+        breakdown[k] = {"total_ms_per_rep": t / reps, "launches_per_rep": n / reps}
     step_ms = t_b * 1e3 / nsteps
     return {"metric": "FP64 adjoint step time (Taylor-4 backward + gradient accumulation)",
             "value": step_ms, "unit": "ms/step", "higher_is_better": False, "n_gpus": 1,
@@ -316,7 +320,7 @@ def bench_adjoint(args, torch, api, _lib, workloads):
                                    f"(np={s.np}), 32 states, dipole x*eps(t), checkpoints every 10"},
             "forward_ms": t_f * 1e3, "backward_ms": t_b * 1e3, "plain_forward_ms": t_plain * 1e3,
             "backward_over_forward_step": t_b / t_plain, "J": f[-1][2],
-            "k_adj_accum_avg_us": acc_ms * 1e3 / acc_n if acc_n else None}
+            "k_adj_accum_avg_us": acc_ms * 1e3 / acc_n if acc_n else None, "kernels_fwd_bwd": breakdown}
 
 
 def bench_batched(args, torch, api, _lib, workloads):

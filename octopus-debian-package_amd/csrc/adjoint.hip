@@ -12,6 +12,7 @@
 // its first-order-in-dt limit.  psi_s is restored from checkpoints every K steps
 // (oct_prop_init/oct_prop_check, propagation.F90:1108-1205, keep states at fixed
 // intervals) and recomputed forward inside each segment, keeping the H^k psi_s terms.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -39,8 +40,10 @@ struct AccumArgs {
 };
 
 // rho(r) = sum_states sum_{m+k<=order-1} zf_{m+k+1} conj(yc_m) yp_k; only Re rho is needed.
-// TP threads per point (a power of two <= 64), each over states t, t+TP, ...
-template <int TP>
+// TP threads per point (a power of two <= 64), each over states t, t+TP, ...; ORD > 0
+// fixes the order at compile time so every (re, im) pair of all 2*ORD arrays is loaded
+// up front as one 16-byte load.
+template <int TP, int ORD>
 __global__ __launch_bounds__(256) void k_adj_accum(const AccumArgs a) {
   __shared__ double wsum[4];
   const int pack = a.L / 2;
@@ -50,12 +53,30 @@ __global__ __launch_bounds__(256) void k_adj_accum(const AccumArgs a) {
   double eps_acc = 0.0;
   for (int r0 = blockIdx.x * GPB; r0 < a.np; r0 += gridDim.x * GPB) {
     const int r = r0 + grp;
+    const bool okr = r < a.np;
+    const size_t row = (size_t)(okr ? r : 0) * a.L;
     double re = 0.0;
-    if (r < a.np) {
-      const size_t row = (size_t)r * a.L;
-      for (int j = t; j < pack; j += TP) {
-        const size_t o = row + 2 * (size_t)j;
-        double sj = 0.0;
+    for (int j = t; j < pack; j += TP) {
+      const size_t o = row + 2 * (size_t)j;
+      double sj = 0.0;
+      if constexpr (ORD > 0) {
+        double2 p[ORD], w[ORD];
+#pragma unroll
+        for (int k = 0; k < ORD; ++k) p[k] = *reinterpret_cast<const double2*>(a.yp[k] + o);
+#pragma unroll
+        for (int m = 0; m < ORD; ++m) w[m] = *reinterpret_cast<const double2*>(a.yc[m] + o);
+#pragma unroll
+        for (int m = 0; m < ORD; ++m) {
+          double ur = 0.0, ui = 0.0;
+#pragma unroll
+          for (int k = 0; k + m < ORD; ++k) {
+            const double cr = a.zr[m + k + 1], ci = a.zi[m + k + 1];
+            ur += cr * p[k].x - ci * p[k].y;
+            ui += cr * p[k].y + ci * p[k].x;
+          }
+          sj += w[m].x * ur + w[m].y * ui;  // Re(conj(w) u)
+        }
+      } else {
         for (int m = 0; m < a.order; ++m) {
           double ur = 0.0, ui = 0.0;
           for (int k = 0; k + m < a.order; ++k) {
@@ -64,15 +85,15 @@ __global__ __launch_bounds__(256) void k_adj_accum(const AccumArgs a) {
             ur += cr * pr - ci * pi;
             ui += cr * pi + ci * pr;
           }
-          const double wr = a.yc[m][o], wi = a.yc[m][o + 1];
-          sj += wr * ur + wi * ui;  // Re(conj(w) u)
+          sj += a.yc[m][o] * ur + a.yc[m][o + 1] * ui;
         }
-        re += sj;
       }
+      re += sj;
     }
+    re = okr ? re : 0.0;
 #pragma unroll
     for (int d = TP / 2; d >= 1; d >>= 1) re += __shfl_xor(re, d, TP);
-    if (t == 0 && r < a.np) {
+    if (t == 0 && okr) {
       a.dv[r] += a.two_vol * re;
       eps_acc += a.xp[r] * re;
     }
@@ -83,6 +104,22 @@ __global__ __launch_bounds__(256) void k_adj_accum(const AccumArgs a) {
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = eps_acc;
   __syncthreads();
   if (threadIdx.x == 0) a.part[blockIdx.x] = ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
+}
+
+template <int ORD>
+int launch_accum(int tp, const AccumArgs& acc, hipStream_t s) {
+  switch (tp) {
+    case 1: hipLaunchKernelGGL((k_adj_accum<1, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    case 2: hipLaunchKernelGGL((k_adj_accum<2, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    case 4: hipLaunchKernelGGL((k_adj_accum<4, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    case 8: hipLaunchKernelGGL((k_adj_accum<8, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    case 16: hipLaunchKernelGGL((k_adj_accum<16, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    case 32: hipLaunchKernelGGL((k_adj_accum<32, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+    default: hipLaunchKernelGGL((k_adj_accum<64, ORD>), dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_adj_accum: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
 }
 
 __global__ __launch_bounds__(256) void k_adj_reduce(const double* __restrict__ part, int n, double scale,
@@ -112,10 +149,11 @@ struct octhip_adjoint_s {
   double dt = 0.0, vol = 0.0;
   double pol[3] = {0.0, 0.0, 0.0};
   double* d_xp = nullptr;       // np
-  double* d_psi = nullptr;      // np_part x L: segment recomputation
-  double* d_ckpt = nullptr;     // nckpt x np_part x L
-  double* d_seg = nullptr;      // K x order x np x L: H^k psi_s of one segment
-  double* d_ychi = nullptr;     // order x np x L: H^m chi_{s+1}
+  double* d_slot = nullptr;     // (K+1) x np x L: psi_s of one segment (recomputed)
+  double* d_ckpt = nullptr;     // nckpt x np x L: psi at every K-th step
+  double* d_seg = nullptr;      // K x (order-1) x np x L: H^k psi_s, k >= 1, of one segment
+  double* d_ychi = nullptr;     // order x np x L: H^m chi_{s+1}, m >= 1 (slot 0 unused)
+  double* d_chi2 = nullptr;     // np x L: the other chi buffer of the backward ping-pong
   double* d_part = nullptr;     // ADJ_GRID
   double* d_deps = nullptr;     // nsteps
   int nckpt = 0;
@@ -127,7 +165,8 @@ namespace {
 
 void free_adjoint(octhip_adjoint_s* a) {
   dfree(a->d_xp);
-  dfree(a->d_psi);
+  dfree(a->d_slot);
+  dfree(a->d_chi2);
   dfree(a->d_ckpt);
   dfree(a->d_seg);
   dfree(a->d_ychi);
@@ -156,7 +195,7 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
   OCTHIP_REQUIRE(order >= 1 && order <= ADJ_MAX_ORDER, "order %d outside 1..%d", order, ADJ_MAX_ORDER);
   OCTHIP_REQUIRE(checkpoint_every >= 1, "checkpoint_every = %d", checkpoint_every);
   OCTHIP_REQUIRE(vol_pp > 0.0, "vol_pp must be positive");
-  const int np = h->np, np_part = h->kin->np_part;
+  const int np = h->np;
   auto* a = new octhip_adjoint_s();
   a->h = h;
   a->ldf = ldf;
@@ -176,9 +215,10 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
     xp[p] = pol[0] * positions[3 * (size_t)p] + pol[1] * positions[3 * (size_t)p + 1] +
             pol[2] * positions[3 * (size_t)p + 2];
   if (!rc) rc = upload(&a->d_xp, xp.data(), xp.size(), nullptr);
-  if (!rc) rc = dalloc((void**)&a->d_psi, rows_bytes(np_part, a->L));
-  if (!rc) rc = dalloc((void**)&a->d_ckpt, rows_bytes((size_t)a->nckpt * np_part, a->L));
-  if (!rc) rc = dalloc((void**)&a->d_seg, rows_bytes((size_t)a->K * order * np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_slot, rows_bytes((size_t)(a->K + 1) * np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_chi2, rows_bytes(np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_ckpt, rows_bytes((size_t)a->nckpt * np, a->L));
+  if (!rc && order > 1) rc = dalloc((void**)&a->d_seg, rows_bytes((size_t)a->K * (order - 1) * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_ychi, rows_bytes((size_t)order * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_part, sizeof(double) * ADJ_GRID);
   if (!rc) rc = dalloc((void**)&a->d_deps, sizeof(double) * nsteps);
@@ -206,11 +246,11 @@ extern "C" int octhip_adjoint_forward(octhip_adjoint_t a, double* psi, const dou
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(a && psi && eps, "null argument");
   hipStream_t s = (hipStream_t)stream;
-  const int np_part = a->h->kin->np_part;
+  const int np = a->h->np;
   for (int i = 0; i < a->nsteps; ++i) a->eps[i] = eps[i];
   for (int st = 0; st < a->nsteps; ++st) {
     if (st % a->K == 0) {
-      OCTHIP_HIP(hipMemcpyAsync(a->d_ckpt + (size_t)(st / a->K) * np_part * a->L, psi, rows_bytes(np_part, a->L),
+      OCTHIP_HIP(hipMemcpyAsync(a->d_ckpt + (size_t)(st / a->K) * np * a->L, psi, rows_bytes(np, a->L),
                                 hipMemcpyDeviceToDevice, s));
     }
     int rc = step_field(a, st, s);
@@ -251,48 +291,52 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
   acc.dv = dj_dv;
   acc.part = a->d_part;
   acc.two_vol = 2.0 * a->vol;
-  for (int m = 0; m < order; ++m) acc.yc[m] = a->d_ychi + m * arr;
+  for (int m = 1; m < order; ++m) acc.yc[m] = a->d_ychi + m * arr;
   double* ysave[ADJ_MAX_ORDER];
+  const int tp = std::min(L / 2, 64);
+  // chi ping-pong: cur holds chi_{s+1}, the step writes chi_s into oth (np inner rows)
+  double* cur = chi;
+  double* oth = a->d_chi2;
   for (int c = a->nckpt - 1; c >= 0; --c) {
     const int s0 = c * a->K, s1 = std::min(s0 + a->K, a->nsteps);
-    // recompute the segment forward from its checkpoint, keeping H^k psi_s
-    OCTHIP_HIP(hipMemcpyAsync(a->d_psi, a->d_ckpt + (size_t)c * np_part * L, rows_bytes(np_part, L),
-                              hipMemcpyDeviceToDevice, s));
+    // recompute the segment from its checkpoint: slot i holds psi_{s0+i}, seg the powers
+    OCTHIP_HIP(hipMemcpyAsync(a->d_slot, a->d_ckpt + (size_t)c * np * L, rows_bytes(np, L), hipMemcpyDeviceToDevice,
+                              s));
     for (int st = s0; st < s1; ++st) {
-      double* base = a->d_seg + (size_t)(st - s0) * order * arr;
-      for (int k = 0; k < order; ++k) ysave[k] = base + k * arr;
+      const int i = st - s0;
+      ysave[0] = a->d_slot + (size_t)i * arr;
+      for (int k = 1; k < order; ++k) ysave[k] = a->d_seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
       int rc = step_field(a, st, s);
-      if (!rc) rc = taylor_step(a->h, a->d_psi, a->ldf, a->dt, order, 0, ysave, s);
+      if (!rc) rc = taylor_step(a->h, a->d_slot + (size_t)(i + 1) * arr, a->ldf, a->dt, order, 0, ysave, s, ysave[0]);
       if (rc) return rc;
     }
     // backward through the segment
     for (int st = s1 - 1; st >= s0; --st) {
-      double* base = a->d_seg + (size_t)(st - s0) * order * arr;
-      for (int k = 0; k < order; ++k) acc.yp[k] = base + k * arr;
-      for (int m = 0; m < order; ++m) ysave[m] = a->d_ychi + m * arr;
+      const int i = st - s0;
+      acc.yp[0] = a->d_slot + (size_t)i * arr;
+      for (int k = 1; k < order; ++k) acc.yp[k] = a->d_seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
+      ysave[0] = cur;
+      for (int m = 1; m < order; ++m) ysave[m] = a->d_ychi + m * arr;
+      acc.yc[0] = cur;
       int rc = step_field(a, st, s);
       // chi_s = p(+i dt H_s) chi_{s+1}, keeping H^m chi_{s+1}
-      if (!rc) rc = taylor_step(a->h, chi, a->ldf, -a->dt, order, 0, ysave, s);
+      if (!rc) rc = taylor_step(a->h, oth, a->ldf, -a->dt, order, 0, ysave, s, cur);
       if (rc) return rc;
-      const int tp = std::min(L / 2, 64);
       {
         ProfScope prof("k_adj_accum", s);
-        switch (tp) {
-          case 1: hipLaunchKernelGGL(k_adj_accum<1>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          case 2: hipLaunchKernelGGL(k_adj_accum<2>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          case 4: hipLaunchKernelGGL(k_adj_accum<4>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          case 8: hipLaunchKernelGGL(k_adj_accum<8>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          case 16: hipLaunchKernelGGL(k_adj_accum<16>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          case 32: hipLaunchKernelGGL(k_adj_accum<32>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-          default: hipLaunchKernelGGL(k_adj_accum<64>, dim3(ADJ_GRID), dim3(256), 0, s, acc); break;
-        }
-        OCTHIP_HIP(hipGetLastError());
+        rc = order == 4 ? launch_accum<4>(tp, acc, s) : launch_accum<0>(tp, acc, s);
+        if (rc) return rc;
       }
       hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, s, a->d_part, ADJ_GRID, 2.0 * a->vol,
                          a->d_deps + st);
       OCTHIP_HIP(hipGetLastError());
+      std::swap(cur, oth);
     }
   }
+  // chi_0 into the caller's buffer, boundary rows zero
+  if (cur != chi) OCTHIP_HIP(hipMemcpyAsync(chi, cur, rows_bytes(np, L), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, s, chi + arr, (size_t)(np_part - np) * L);
+  OCTHIP_HIP(hipGetLastError());
   OCTHIP_HIP(hipMemcpyAsync(dj_deps, a->d_deps, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost, s));
   OCTHIP_HIP(hipStreamSynchronize(s));
   return OCTHIP_SUCCESS;

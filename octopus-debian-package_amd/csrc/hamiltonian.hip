@@ -425,7 +425,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
   int rc = prepare_taylor(h, ldf);
   if (rc) return rc;
-  return taylor_step(h, psi, ldf, deltat, order, imag_time, nullptr, (hipStream_t)stream);
+  return taylor_step(h, psi, ldf, deltat, order, imag_time, nullptr, (hipStream_t)stream, nullptr);
 }
 
 // ---------------------------------------------------------------- graphs --
@@ -448,7 +448,7 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
                        [&](hipStream_t cs) {
                          int r = OCTHIP_SUCCESS;
                          for (int b = 0; b < nbatch && !r; ++b)
-                           r = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs);
+                           r = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs, nullptr);
                          return r;
                        },
                        out);
@@ -514,25 +514,30 @@ namespace octhip {
 // save[0] = psi (boundary cleared) and save[k] = H^k psi for k = 1..order-1: the terms the
 // discrete adjoint contracts with the backward states.
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                double* const* save, hipStream_t s) {
+                double* const* save, hipStream_t s, const double* psi_in) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
   int rc = OCTHIP_SUCCESS;
-  // psi's boundary rows are cleared by the first bra launch (or here when there is no
-  // nonlocal term); every pass reads boundary rows as zero (lim = np) in any case
-  if (h->nmat == 0) {
+  const double* src = psi_in ? psi_in : psi;
+  // in place: psi's boundary rows are cleared by the first bra launch (or here when there
+  // is no nonlocal term); every pass reads boundary rows as zero (lim = np) in any case.
+  // Out of place (psi_in): only the np inner rows of psi are written.
+  if (!psi_in && h->nmat == 0) {
     rc = zero_boundary(psi, op->np, op->np_part, L, s);
     if (rc) return rc;
   }
-  if (save) OCTHIP_HIP(hipMemcpyAsync(save[0], psi, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
+  if (save && save[0] != src)
+    OCTHIP_HIP(hipMemcpyAsync(save[0], src, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
+  if (psi_in && order == 1)  // the order-1 axpy works in place on psi
+    OCTHIP_HIP(hipMemcpyAsync(psi, psi_in, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
   std::complex<double> zf[33];
   taylor_coefs(deltat, imag_time, order, zf);
   std::complex<double> z, z1;
   for (int k = 1; k <= order; ++k) {
     z = zf[k];
-    const double* yin = (k == 1) ? psi : (save ? save[k - 1] : h->d_y[(k - 2) & 1]);
+    const double* yin = (k == 1) ? src : (save ? save[k - 1] : h->d_y[(k - 2) & 1]);
     double* yout = (k < order) ? (save ? save[k] : h->d_y[(k - 1) & 1]) : nullptr;
-    rc = run_bra(h, yin, L, s, (k == 1 && h->nmat > 0) ? psi : nullptr);
+    rc = run_bra(h, yin, L, s, (k == 1 && h->nmat > 0 && !psi_in) ? psi : nullptr);
     if (rc) return rc;
     ApplyArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -557,6 +562,7 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
       z1 = z;
     } else {
       a.acc = psi;
+      a.acc_in = (k == 2) ? psi_in : nullptr;  // from the third order on psi holds the sum
       a.has_c1 = (k == 2);
       a.c1r = z1.real();
       a.c1i = z1.imag();

@@ -40,9 +40,11 @@ struct ApplyArgs {
   // nonlocal ket: kslot[p] = row of ksum holding sum_a P_a proj_a at point p, or -1
   const int* __restrict__ kslot;
   const double* __restrict__ ksum;
-  // Taylor epilogue: yout = Hy ; acc += c1 * y + c2 * Hy (complex lanes)
+  // Taylor epilogue: yout = Hy ; acc = acc_in + c1 * y + c2 * Hy (complex lanes), acc_in
+  // = acc when null (in place)
   double* yout;
   double* acc;
+  const double* acc_in;
   double c1r, c1i, c2r, c2i;
   int has_c1;
 };
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const ApplyArgs a) {
         static_assert(EPI != EPI_TAYLOR || LPT == 2, "Taylor epilogue needs complex lane pairs");
         if (a.yout) stv<LPT>(a.yout + (size_t)p * L + l, s);
         if (a.acc) {
-          Vec<LPT> av = ldv<LPT>(a.acc + (size_t)p * L + l);
+          Vec<LPT> av = ldv<LPT>((a.acc_in ? a.acc_in : a.acc) + (size_t)p * L + l);
           if (a.has_c1) {
             av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], x.v[1], false);
             av.v[1] = av.v[1] + zmul_lane(a.c1r, a.c1i, x.v[1], x.v[0], true);

@@ -170,8 +170,9 @@ struct octhip_hamiltonian_s {
 namespace octhip {
 // Taylor building blocks shared by the propagator entry points and the adjoint.
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf);
+// psi_in (optional): take psi_s from psi_in and write psi_{s+1} to the np inner rows of psi
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                double* const* save, hipStream_t s);
+                double* const* save, hipStream_t s, const double* psi_in = nullptr);
 void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z);
 // launches of body(capturing stream) as one instantiated graph (octhip_taylor_graph_launch)
 int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out);

@@ -63,6 +63,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const double* const a_ksum = A.ksum;
   double* const a_yout = A.yout;
   double* const a_acc = A.acc;
+  const double* const a_acc_in = A.acc_in ? A.acc_in : A.acc;
   const double a_c1r = A.c1r, a_c1i = A.c1i, a_c2r = A.c2r, a_c2i = A.c2i;
   const int a_has_c1 = A.has_c1;
   double a_w[25];
@@ -227,7 +228,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       const int ks = tks[(P - zb + 4) * 64 + col];
       kr[slot] = ldv<LPT>((ok && ks >= 0) ? a_ksum + (size_t)ks * L + l : a_zrow);
     }
-    if constexpr (EPI == EPI_TAYLOR) ar[slot] = ldv<LPT>((has_acc && ok) ? a_acc + (size_t)q * L + l : a_zrow);
+    if constexpr (EPI == EPI_TAYLOR) ar[slot] = ldv<LPT>((has_acc && ok) ? a_acc_in + (size_t)q * L + l : a_zrow);
   };
 #pragma unroll
   for (int k = 0; k < 10; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
