@@ -22,7 +22,8 @@
 
 namespace octhip {
 void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a);
-int zmarch_items(octhip_nl_operator_s* op);
+int zmarch_items(octhip_nl_operator_s* op, int zs);
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk);
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
                         hipStream_t s);
 }  // namespace octhip
@@ -40,6 +41,8 @@ struct Pair {
   double* proj = nullptr;
   double* ksum = nullptr;
   double* y[2] = {nullptr, nullptr};
+  const int4* items = nullptr;   // the operator's z-march items for this pair's segment length
+  int nitems = 0;
 };
 
 // a device table of argument blocks plus block offsets for one multi-segment kernel
@@ -75,7 +78,7 @@ struct octhip_batched_s {
   int order = 4;
   double dt = 0.0;
   bool mfma = true, nl = false;
-  int lc = 16, lpt = 2, zseg = 16;
+  int lc = 16, lpt = 2, zseg = 8;
   std::complex<double> zf[33];
   Seg zero, reduce, ket, ksum;
   std::vector<Seg> bra;                 // [0] H-apply input psi, [k] Taylor order k input
@@ -164,7 +167,7 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
     z.mcube = op->d_mcube;
     z.mx = op->mx;
     z.my = op->my;
-    z.items = op->d_items;
+    z.items = p.items;
     z.sched = nullptr;
     // every pointer the kernel dereferences must exist
     // (the H-apply descriptor may lack fo: octhip_batched_apply refuses to run without hpsi)
@@ -241,9 +244,13 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     }
     if (!rc) rc = dalloc((void**)&p.y[0], sizeof(double) * h->np * L);
     if (!rc) rc = dalloc((void**)&p.y[1], sizeof(double) * h->np * L);
+    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc));
+    if (!rc) {
+      p.items = h->kin->d_items;
+      p.nitems = h->kin->nitems;
+      b->zseg = std::max(b->zseg, h->kin->zseg_built);
+    }
     b->pairs.push_back(p);
-    if (!rc) rc = zmarch_items(h->kin);
-    if (!rc) b->zseg = std::max(b->zseg, h->kin->zseg_built);
   }
   // boundary clear, reduce, ket, ket sums
   if (!rc) {
@@ -309,10 +316,9 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   if (!rc) {
     std::vector<int4> all;
     for (size_t i = 0; i < b->pairs.size(); ++i) {
-      const octhip_nl_operator_s* op = b->pairs[i].h->kin;
       const int nchunk = b->pairs[i].L / b->lc;
       for (int c = 0; c < nchunk; ++c)
-        for (int it = 0; it < op->nitems; ++it) all.push_back(make_int4((int)i, it, c, 0));
+        for (int it = 0; it < b->pairs[i].nitems; ++it) all.push_back(make_int4((int)i, it, c, 0));
     }
     const int n = (int)all.size();
     const int per = (n + 7) / 8;

@@ -153,34 +153,67 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
 }  // namespace
 
 namespace octhip {
-// (tile, z-segment) work items for the current segment length
-int zmarch_items(octhip_nl_operator_s* op) {
-  if (op->zseg_built == op->zseg && op->d_items) return OCTHIP_SUCCESS;
+// Segment length of a launch with nchunk lane chunks: the operator's setting, or 32 planes
+// halved (down to 8) while the launch would have fewer than 1024 workgroups (measured on
+// MI355X: 32 is fastest once there are a few workgroups per CU; short segments cost
+// overlap reads but keep small meshes from leaving CUs idle).
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk) {
+  if (op->zseg > 0) return op->zseg;
+  int zs = 32;
+  for (; zs > 8; zs /= 2) {
+    long n = 0;
+    for (size_t t = 0; t < op->tile_zlo.size(); ++t)
+      if (op->tile_zhi[t] >= 0) n += (op->tile_zhi[t] - op->tile_zlo[t] + zs) / zs;
+    if (n * nchunk >= 1024) break;
+  }
+  return zs;
+}
+
+// (tile, z-segment) work items for segment length zs; every list built stays allocated
+// until the operator is destroyed (graphs and batched launches may point at it)
+int zmarch_items(octhip_nl_operator_s* op, int zs) {
+  if (op->zseg_built == zs && op->d_items) return OCTHIP_SUCCESS;
+  // save the current schedules with their plan, then switch
+  for (auto& pl : op->plans)
+    if (pl.zseg == op->zseg_built) {
+      pl.sched = op->sched;
+      pl.sched_len = op->sched_len;
+    }
+  for (auto& pl : op->plans)
+    if (pl.zseg == zs) {
+      op->d_items = pl.d_items;
+      op->items = pl.items;
+      op->nitems = (int)pl.items.size();
+      op->sched = pl.sched;
+      op->sched_len = pl.sched_len;
+      op->zseg_built = zs;
+      return OCTHIP_SUCCESS;
+    }
   std::vector<int4> items;
   // segment-major, then tile rows: concurrently running items share halo rows in L2
   int maxlen = 0;
   for (size_t t = 0; t < op->tile_zlo.size(); ++t)
     if (op->tile_zhi[t] >= 0) maxlen = std::max(maxlen, op->tile_zhi[t] - op->tile_zlo[t] + 1);
-  for (int sg = 0; sg * op->zseg < maxlen; ++sg)
+  for (int sg = 0; sg * zs < maxlen; ++sg)
     for (int ty = 0; ty < op->nty; ++ty)
       for (int tx = 0; tx < op->ntx; ++tx) {
         const int t = tx + op->ntx * ty;
         if (op->tile_zhi[t] < 0) continue;
-        const int zb = op->tile_zlo[t] + sg * op->zseg;
+        const int zb = op->tile_zlo[t] + sg * zs;
         if (zb > op->tile_zhi[t]) continue;
-        const int ze = std::min(zb + op->zseg, op->tile_zhi[t] + 1);
+        const int ze = std::min(zb + zs, op->tile_zhi[t] + 1);
         items.push_back(make_int4(tx, ty, zb, ze));
       }
-  if (op->d_items) op->retired.push_back(op->d_items);
-  op->d_items = nullptr;
-  for (auto& e : op->sched) op->retired.push_back(e.second);
-  op->sched.clear();
-  op->sched_len.clear();
+  int4* d = nullptr;
+  int rc = upload(&d, items.data(), items.size(), nullptr);
+  if (rc) return rc;
+  op->plans.push_back({zs, d, items, {}, {}});
+  op->d_items = d;
   op->items = items;
   op->nitems = (int)items.size();
-  int rc = upload(&op->d_items, items.data(), items.size(), nullptr);
-  if (rc) return rc;
-  op->zseg_built = op->zseg;
+  op->sched.clear();
+  op->sched_len.clear();
+  op->zseg_built = zs;
   return OCTHIP_SUCCESS;
 }
 
@@ -207,6 +240,11 @@ int zmarch_sched(octhip_nl_operator_s* op, int nchunk, const int2** out, int* le
   if (rc) return rc;
   op->sched.push_back({nchunk, d});
   op->sched_len.push_back((int)sc.size());
+  for (auto& pl : op->plans)
+    if (pl.zseg == op->zseg_built) {
+      pl.sched = op->sched;
+      pl.sched_len = op->sched_len;
+    }
   *out = d;
   *len = (int)sc.size();
   return OCTHIP_SUCCESS;
@@ -219,10 +257,10 @@ int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, 
 int prepare_operator(octhip_nl_operator_s* op, int L) {
   const bool zm = op->zmarch_ok && op->kernel != 1 && op->n == 25;
   if (!zm) return OCTHIP_SUCCESS;
-  int rc = zmarch_items(op);
-  if (rc) return rc;
   int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
   if (lc > L) lc = L;
+  int rc = zmarch_items(op, zmarch_zseg(op, L / lc));
+  if (rc) return rc;
   const int2* sched = nullptr;
   int len = 0;
   return zmarch_sched(op, L / lc, &sched, &len);
@@ -235,11 +273,11 @@ int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t
     return fail(OCTHIP_ERR_STATE, "apply: incomplete kernel arguments");
   const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25;
   if (!zm) return launch_apply(a, op->lattice && a.n > 0, epi, s);
-  int rc = zmarch_items(op);
-  if (rc) return rc;
-  if (op->nitems == 0 || a.np <= 0) return OCTHIP_SUCCESS;
   int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
   if (lc > a.L) lc = a.L;
+  int rc = zmarch_items(op, zmarch_zseg(op, a.L / lc));
+  if (rc) return rc;
+  if (op->nitems == 0 || a.np <= 0) return OCTHIP_SUCCESS;
   int lpt = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
   if (lpt > lc) lpt = lc;
   if (lc == 32) lpt = 2;  // 1024 threads per workgroup at most
@@ -336,8 +374,12 @@ extern "C" int octhip_nl_operator_destroy(octhip_nl_operator_t op) {
   dfree(op->d_cube);
   dfree(op->d_dcube);
   dfree(op->d_mcube);
-  dfree(op->d_items);
-  for (auto& e : op->sched) dfree(e.second);
+  for (auto& pl : op->plans)
+    if (pl.zseg == op->zseg_built) pl.sched = op->sched;
+  for (auto& pl : op->plans) {
+    dfree(pl.d_items);
+    for (auto& e : pl.sched) dfree(e.second);
+  }
   for (void* p : op->retired) dfree(p);
   dfree(op->d_ri);
   dfree(op->d_rimap);
@@ -362,7 +404,7 @@ extern "C" int octhip_nl_operator_set_tuning(octhip_nl_operator_t op, int key, i
       op->tile_iters = value;
       return OCTHIP_SUCCESS;
     case OCTHIP_TUNE_ZSEG:
-      OCTHIP_REQUIRE(value >= 1 && value <= 64, "z segment %d outside 1..64", value);
+      OCTHIP_REQUIRE(value >= 0 && value <= 64, "z segment %d outside 0..64 (0 = automatic)", value);
       op->zseg = value;
       return OCTHIP_SUCCESS;
     case OCTHIP_TUNE_LANE_CHUNK:

@@ -105,7 +105,7 @@ struct octhip_nl_operator_s {
   bool zmarch_ok = false;
   int* d_mcube = nullptr;
   int mx = 0, my = 0, mz = 0;
-  int zseg = 32;                         // output planes per work item (sweep on MI355X: 32 best)
+  int zseg = 0;                          // output planes per work item, 0 = chosen per launch
   int lane_chunk = 0;                    // lanes per workgroup (0 = automatic)
   int lanes_per_thread = 0;              // doubles per thread (0 = automatic)
   int zseg_built = 0;
@@ -116,6 +116,16 @@ struct octhip_nl_operator_s {
   std::vector<int4> items;
   std::vector<std::pair<int, int2*>> sched;  // lane chunks -> device schedule
   std::vector<int> sched_len;
+  // item lists built so far, one per segment length (the current one is mirrored in
+  // d_items / nitems / items / sched / zseg_built)
+  struct ZPlan {
+    int zseg;
+    int4* d_items;
+    std::vector<int4> items;
+    std::vector<std::pair<int, int2*>> sched;
+    std::vector<int> sched_len;
+  };
+  std::vector<ZPlan> plans;
   // device buffers replaced by a rebuild stay allocated until destroy: captured graphs and
   // batched launches may still point at them
   std::vector<void*> retired;

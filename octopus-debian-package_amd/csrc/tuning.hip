@@ -168,7 +168,7 @@ extern "C" int octhip_tuning_load(const char* path) {
       std::fclose(f);
       return fail(OCTHIP_ERR_INVALID, "%s: malformed line %d", path, n + 1);
     }
-    const bool ok = v.kernel >= 0 && v.kernel <= 2 && v.zseg >= 1 && v.zseg <= 64 && v.gather_tile >= 1 &&
+    const bool ok = v.kernel >= 0 && v.kernel <= 2 && v.zseg >= 0 && v.zseg <= 64 && v.gather_tile >= 1 &&
                     v.gather_tile <= 8 && (v.bra == 0 || v.bra == 1) && v.lpt >= 0 && v.lpt <= 2;
     if (!ok) {
       std::fclose(f);
