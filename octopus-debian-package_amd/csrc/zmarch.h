@@ -7,9 +7,13 @@
 //     (double-buffered): x and y neighbours are LDS reads;
 //   * each thread keeps its column's values at z-4..z+4 in registers: z neighbours cost
 //     nothing;
-//   * loads run two planes ahead (halo values), six ahead (own column values) and one
-//     more for the site -> point indices, into ring buffers with compile-time slots (the
-//     z loop is unrolled by the ring period), so a step waits only on old loads.
+//   * the site -> point indices of the item (and the ket slots of its own sites) are
+//     staged in LDS once per item;
+//   * loads run ZM_HALO_AHEAD planes ahead (halo values), six ahead (own column values)
+//     and ZM_AUX_AHEAD ahead (potential, ket row, Taylor accumulator), into ring buffers
+//     with compile-time slots (the z loop is unrolled by the ring period); every load is
+//     unconditional, so the compiler's wait counts stay exact and a step waits only on
+//     loads issued planes earlier.
 // Per output point the tile reads 1 own row chunk + 2 halo chunks (128 halo sites for 64
 // columns) instead of 25 rows.  The lattice is a padded "march cube" of point indices;
 // sites that are not points hold -1.  Values follow the gather kernel exactly: point
