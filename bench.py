@@ -106,12 +106,25 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
         batches.append(api.Batch.from_states(z, s.np_part))
         outs.append(api.Batch(pack, s.np, True))
     torch.cuda.synchronize()
+    # batch b runs on stream b mod --streams, as the reference's propagation loop sets one
+    # stream per batch (td/propagation_ops_elec.F90:236); every stream has its own work set
+    main = torch.cuda.current_stream()
+    streams = [main] if args.streams <= 1 else [torch.cuda.Stream() for _ in range(args.streams)]
 
     def step():
-        for pb, hb in zip(batches, outs):
-            api.hamiltonian_elec_apply_batch(hm, pb, hb)
+        if len(streams) == 1:
+            for pb, hb in zip(batches, outs):
+                api.hamiltonian_elec_apply_batch(hm, pb, hb)
+            return
+        for st in streams:
+            st.wait_stream(main)
+        for i, (pb, hb) in enumerate(zip(batches, outs)):
+            with torch.cuda.stream(streams[i % len(streams)]):
+                api.hamiltonian_elec_apply_batch(hm, pb, hb)
+        for st in streams:
+            main.wait_stream(st)
 
-    log(f"np={s.np} np_part={s.np_part} warmup={args.warmup} steps={args.steps}")
+    log(f"np={s.np} np_part={s.np_part} warmup={args.warmup} steps={args.steps} streams={len(streams)}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -153,7 +166,8 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
         "config": {"workload": "complex H-apply, sphere R=12 h=0.25 (np=462781, np_part=564189), "
                                "order-4 star, 64 states as 2 packed batches of 32, 16 wells, "
                                "16 atoms x 4 KB projectors (rc=3)",
-                   "states": nst_total, "pack_size": pack, "np": s.np, "np_part": s.np_part},
+                   "states": nst_total, "pack_size": pack, "np": s.np, "np_part": s.np_part,
+                   "streams": len(streams)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": args.traffic if args.traffic is not None else profiled_traffic(),
@@ -406,6 +420,7 @@ def main() -> int:
     ap.add_argument("--lane-chunk", type=int, default=None)
     ap.add_argument("--lanes-per-thread", type=int, default=None)
     ap.add_argument("--bra", type=int, default=1)
+    ap.add_argument("--streams", type=int, default=1, help="H-apply: batches spread over this many streams")
     ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

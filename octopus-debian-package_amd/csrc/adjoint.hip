@@ -222,7 +222,7 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
   if (!rc) rc = dalloc((void**)&a->d_ychi, rows_bytes((size_t)order * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_part, sizeof(double) * ADJ_GRID);
   if (!rc) rc = dalloc((void**)&a->d_deps, sizeof(double) * nsteps);
-  if (!rc) rc = prepare_taylor(h, ldf);
+  if (!rc) rc = prepare_taylor(h, ldf, nullptr);
   if (!rc) {
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "adjoint_create: %s", hipGetErrorString(e));

@@ -367,7 +367,7 @@ extern "C" int octhip_batched_graph_create(octhip_batched_t b, void* stream, oct
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(b && out, "null argument");
   *out = nullptr;
-  return capture_graph((hipStream_t)stream, [&](hipStream_t cs) { return octhip_batched_taylor_step(b, cs); }, out);
+  return capture_graph((hipStream_t)stream, nullptr, [&](hipStream_t cs) { return octhip_batched_taylor_step(b, cs); }, out);
 }
 
 extern "C" int octhip_batched_destroy(octhip_batched_t b) {

@@ -19,35 +19,59 @@ namespace {
 
 enum { O_POINTS = 0, O_PROJS, O_MATRIX, O_MAP, O_SCAL, O_MIX };
 
-int ensure_work(octhip_hamiltonian_s* h, int ldf) {
-  if (h->work_ldf >= ldf) return OCTHIP_SUCCESS;
-  for (void* p : {(void*)h->d_partial, (void*)h->d_proj, (void*)h->d_ksum})
-    if (p) h->retired.push_back(p);
-  h->d_partial = h->d_proj = h->d_ksum = nullptr;
-  const size_t L = (size_t)1 << ldf;
-  int rc = OCTHIP_SUCCESS;
-  if (h->nmat > 0) {
-    rc = dalloc((void**)&h->d_partial, sizeof(double) * h->nchunk * h->max_nprojs * L);
-    if (!rc) rc = dalloc((void**)&h->d_proj, sizeof(double) * h->full_projection_size * L);
-    if (!rc) rc = dalloc((void**)&h->d_ksum, sizeof(double) * ((size_t)h->total_points + h->nmulti) * L);
+// the work-buffer set of stream s, created empty on first use, and mirrored into h
+octhip_hamiltonian_s::WorkSet& workset(octhip_hamiltonian_s* h, hipStream_t s) {
+  for (auto& w : h->sets)
+    if (w.s == s) return w;
+  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr});
+  return h->sets.back();
+}
+
+void bind(octhip_hamiltonian_s* h, const octhip_hamiltonian_s::WorkSet& w) {
+  h->d_partial = w.partial;
+  h->d_proj = w.proj;
+  h->d_ksum = w.ksum;
+  h->d_y[0] = w.y0;
+  h->d_y[1] = w.y1;
+  h->work_ldf = w.work_ldf;
+  h->y_ldf = w.y_ldf;
+}
+
+// projector work buffers of stream s for rows of 2^ldf lanes (outgrown ones are retired)
+int ensure_work(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
+  auto& w = workset(h, s);
+  if (w.work_ldf < ldf) {
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum})
+      if (p) h->retired.push_back(p);
+    w.partial = w.proj = w.ksum = nullptr;
+    const size_t L = (size_t)1 << ldf;
+    int rc = OCTHIP_SUCCESS;
+    if (h->nmat > 0) {
+      rc = dalloc((void**)&w.partial, sizeof(double) * h->nchunk * h->max_nprojs * L);
+      if (!rc) rc = dalloc((void**)&w.proj, sizeof(double) * h->full_projection_size * L);
+      if (!rc) rc = dalloc((void**)&w.ksum, sizeof(double) * ((size_t)h->total_points + h->nmulti) * L);
+    }
+    if (rc) return rc;
+    w.work_ldf = ldf;
   }
-  if (rc) return rc;
-  h->work_ldf = ldf;
+  bind(h, w);
   return OCTHIP_SUCCESS;
 }
 
-int ensure_taylor(octhip_hamiltonian_s* h, int ldf) {
-  if (h->y_ldf >= ldf) return OCTHIP_SUCCESS;
-  for (auto& y : h->d_y) {
-    if (y) h->retired.push_back(y);
-    y = nullptr;
-  }
-  const size_t bytes = sizeof(double) * ((size_t)h->np << ldf);
-  for (auto& y : h->d_y) {
-    int rc = dalloc((void**)&y, bytes);
+// Taylor intermediates of stream s (np rows each)
+int ensure_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
+  auto& w = workset(h, s);
+  if (w.y_ldf < ldf) {
+    for (void* p : {(void*)w.y0, (void*)w.y1})
+      if (p) h->retired.push_back(p);
+    w.y0 = w.y1 = nullptr;
+    const size_t bytes = sizeof(double) * ((size_t)h->np << ldf);
+    int rc = dalloc((void**)&w.y0, bytes);
+    if (!rc) rc = dalloc((void**)&w.y1, bytes);
     if (rc) return rc;
+    w.y_ldf = ldf;
   }
-  h->y_ldf = ldf;
+  bind(h, w);
   return OCTHIP_SUCCESS;
 }
 
@@ -279,7 +303,6 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_kslot);
   dfree(h->d_multi_pos);
   dfree(h->d_multi_g);
-  dfree(h->d_ksum);
   dfree(h->d_v);
   dfree(h->d_xyz);
   dfree(h->d_veff);
@@ -293,11 +316,9 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_mat_npj);
   dfree(h->d_chunks);
   dfree(h->d_mat_chunk0);
-  dfree(h->d_partial);
-  dfree(h->d_proj);
+  for (auto& w : h->sets)
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1}) dfree(p);
   for (void* p : h->retired) dfree(p);
-  dfree(h->d_y[0]);
-  dfree(h->d_y[1]);
   delete h;
   return OCTHIP_SUCCESS;
 }
@@ -353,7 +374,7 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   hipStream_t s = (hipStream_t)stream;
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
-  int rc = ensure_work(h, ldf);
+  int rc = ensure_work(h, ldf, s);
   if (rc) return rc;
   // boundaries_set: the boundary rows are cleared inside the bra launch when there is one
   if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
@@ -403,9 +424,9 @@ void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double
   if (a.kslot) a.ksum = ksum;
 }
 
-int prepare_taylor(octhip_hamiltonian_s* h, int ldf) {
-  int rc = ensure_work(h, ldf);
-  if (!rc) rc = ensure_taylor(h, ldf);
+int prepare_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
+  int rc = ensure_work(h, ldf, s);
+  if (!rc) rc = ensure_taylor(h, ldf, s);
   if (!rc) rc = prepare_operator(h->kin, 1 << ldf);
   return rc;
 }
@@ -423,7 +444,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
   OCTHIP_REQUIRE(h && psi, "null argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
-  int rc = prepare_taylor(h, ldf);
+  int rc = prepare_taylor(h, ldf, (hipStream_t)stream);
   if (rc) return rc;
   return taylor_step(h, psi, ldf, deltat, order, imag_time, nullptr, (hipStream_t)stream, nullptr);
 }
@@ -442,9 +463,12 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
   *out = nullptr;
-  int rc = prepare_taylor(h, ldf);
-  if (rc) return rc;
-  return capture_graph((hipStream_t)stream,
+  hipStream_t used = nullptr;
+  int rc = capture_graph((hipStream_t)stream,
+                       [&](hipStream_t cs) {
+                         used = cs;
+                         return prepare_taylor(h, ldf, cs);
+                       },
                        [&](hipStream_t cs) {
                          int r = OCTHIP_SUCCESS;
                          for (int b = 0; b < nbatch && !r; ++b)
@@ -452,16 +476,27 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
                          return r;
                        },
                        out);
+  // the graph owns the buffers it was captured with: later calls on any stream get others
+  pin_workset(h, used);
+  return rc;
 }
 
 namespace octhip {
 // Capture the launches `body` issues on a capturing stream into an instantiated graph.
-int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out) {
+int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
+                  const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out) {
   hipStream_t cs = s;
   bool own = false;
   if (!cs) {  // the legacy default stream cannot be captured
     OCTHIP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     own = true;
+  }
+  if (prepare) {
+    int prc = prepare(cs);
+    if (prc) {
+      if (own) (void)hipStreamDestroy(cs);
+      return prc;
+    }
   }
   OCTHIP_HIP(hipStreamSynchronize(cs));
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -491,6 +526,11 @@ int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, oc
   *out = g;
   return OCTHIP_SUCCESS;
 }
+
+void pin_workset(octhip_hamiltonian_s* h, hipStream_t s) {
+  for (auto& w : h->sets)
+    if (w.s == s) w.s = reinterpret_cast<hipStream_t>(static_cast<intptr_t>(-(++h->graph_sets)));
+}
 }  // namespace octhip
 
 extern "C" int octhip_taylor_graph_launch(octhip_taylor_graph_t g, void* stream) {
@@ -517,7 +557,8 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
                 double* const* save, hipStream_t s, const double* psi_in) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
-  int rc = OCTHIP_SUCCESS;
+  int rc = prepare_taylor(h, ldf, s);  // the work buffers of stream s (allocated before a capture)
+  if (rc) return rc;
   const double* src = psi_in ? psi_in : psi;
   // in place: psi's boundary rows are cleared by the first bra launch (or here when there
   // is no nonlocal term); every pass reads boundary rows as zero (lim = np) in any case.

@@ -166,11 +166,21 @@ struct octhip_hamiltonian_s {
   int* d_kslot = nullptr;       // np: row of the ket buffer for each mesh point, or -1
   int* d_multi_pos = nullptr;   // nmulti+1
   int* d_multi_g = nullptr;     // sphere points of the multi-sphere mesh points
-  double* d_ksum = nullptr;     // (total_points + nmulti) * L ket rows
-  // work buffers sized for the largest ldf seen
+  // Work buffers, one set per stream: applies on different streams (Octopus gives every
+  // batch its own stream, accel_set_stream) may run concurrently.  The set of the stream
+  // of the current call is mirrored in the fields below; sets captured into a graph are
+  // re-keyed so no later stream reuses them.
+  struct WorkSet {
+    hipStream_t s;
+    int work_ldf, y_ldf;
+    double *partial, *proj, *ksum, *y0, *y1;
+  };
+  std::vector<WorkSet> sets;
+  long graph_sets = 0;
   int work_ldf = -1;
   double* d_partial = nullptr;  // nchunk * max_nprojs * L
   double* d_proj = nullptr;     // full_projection_size * L
+  double* d_ksum = nullptr;     // (total_points + nmulti) * L ket rows
   double* d_y[2] = {nullptr, nullptr};  // Taylor intermediates, np rows each
   int y_ldf = -1;
   std::vector<void*> retired;   // work buffers outgrown by a larger ldf, freed at destroy
@@ -179,13 +189,17 @@ struct octhip_hamiltonian_s {
 
 namespace octhip {
 // Taylor building blocks shared by the propagator entry points and the adjoint.
-int prepare_taylor(octhip_hamiltonian_s* h, int ldf);
+int prepare_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s);
 // psi_in (optional): take psi_s from psi_in and write psi_{s+1} to the np inner rows of psi
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
                 double* const* save, hipStream_t s, const double* psi_in = nullptr);
 void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z);
 // launches of body(capturing stream) as one instantiated graph (octhip_taylor_graph_launch)
-int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out);
+// prepare(capturing stream) runs before the capture starts (allocations go there)
+int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
+                  const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out);
+// the work-buffer set of stream s is taken out of the per-stream lookup (after a capture)
+void pin_workset(octhip_hamiltonian_s* h, hipStream_t s);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
 int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
 }  // namespace octhip
