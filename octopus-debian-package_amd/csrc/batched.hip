@@ -24,6 +24,7 @@ namespace octhip {
 void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a);
 int zmarch_items(octhip_nl_operator_s* op, int zs);
 int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk);
+bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a);
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
                         hipStream_t s);
 }  // namespace octhip
@@ -174,6 +175,9 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
     const bool ok = z.a.fi && z.a.zrow && z.mcube && z.items &&
                     (!z.a.kslot || z.a.ksum) && (k < 2 || z.a.acc) && (k == 0 || k == b->order || z.a.yout);
     if (!ok) return fail(OCTHIP_ERR_STATE, "batched: incomplete descriptor for pair %zu, pass %d", i, k);
+    if (!zmarch_fits(op, z.a))
+      return fail(OCTHIP_ERR_INVALID, "batched: pair %zu is too large for 32-bit row offsets (%d rows of %d doubles)",
+                  i, op->np_part, p.L);
   }
   return upload(out, zs.data(), zs.size(), nullptr);
 }

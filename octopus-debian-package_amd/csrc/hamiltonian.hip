@@ -176,6 +176,7 @@ void fill_terms(const octhip_hamiltonian_s* h, int terms, ApplyArgs& a) {
   if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0 && h->nslot > 0) {
     a.kslot = h->d_kslot;
     a.ksum = h->d_ksum;
+    a.nk = h->total_points + h->nmulti;
   }
 }
 

@@ -40,6 +40,7 @@ struct ApplyArgs {
   // nonlocal ket: kslot[p] = row of ksum holding sum_a P_a proj_a at point p, or -1
   const int* __restrict__ kslot;
   const double* __restrict__ ksum;
+  int nk;                          // rows of ksum
   // Taylor epilogue: yout = Hy ; acc = acc_in + c1 * y + c2 * Hy (complex lanes), acc_in
   // = acc when null (in place)
   double* yout;

@@ -252,6 +252,15 @@ int zmarch_sched(octhip_nl_operator_s* op, int nchunk, const int2** out, int* le
 
 int launch_zmarch(const ZArgs& z, int lc, int lpt, int epi, int grid, int zseg, hipStream_t s);
 
+// The z-march addresses rows with 32-bit byte offsets: every row array it touches (psi
+// over np_part rows, the outputs over np, the ket rows) must satisfy
+// (rows + 1) * row bytes <= 2^32; larger batches take the gather kernel.
+bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a) {
+  const uint64_t rowb = (uint64_t)a.L * sizeof(double);
+  const uint64_t lim = 1ull << 32;
+  return (uint64_t)(op->np_part + 1) * rowb <= lim && (!a.kslot || (uint64_t)(a.nk + 1) * rowb <= lim);
+}
+
 // Build every lazily created device structure an apply with rows of L doubles needs, so
 // that later applies allocate nothing (required before a graph capture).
 int prepare_operator(octhip_nl_operator_s* op, int L) {
@@ -271,7 +280,7 @@ int prepare_operator(octhip_nl_operator_s* op, int L) {
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s) {
   if (!a.fi || !a.zrow || (a.kslot && !a.ksum) || (epi == EPI_STORE && !a.fo))
     return fail(OCTHIP_ERR_STATE, "apply: incomplete kernel arguments");
-  const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25;
+  const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25 && zmarch_fits(op, a);
   if (!zm) return launch_apply(a, op->lattice && a.n > 0, epi, s);
   int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
   if (lc > a.L) lc = a.L;

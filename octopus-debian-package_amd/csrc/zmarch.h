@@ -37,6 +37,39 @@ constexpr int ZPADZ = 8;             // march-cube padding below/above the inner
 #define ZM_AUX_AHEAD 4                 // planes of output-point data loaded ahead
 #endif
 
+// Raw buffer resources over the row arrays (gfx9 range checking, stride 0): a load whose
+// byte offset lies at or beyond num_records returns zeros and such a store is dropped.
+// Offsets are 32-bit, row q at q << log2(row bytes), so the point index -1 (and any
+// index >= the array's row count) falls out of range: this replaces the zero-row pointer
+// selects and keeps every load and store unconditional.  The host sends a batch here only
+// when (rows + 1) * row bytes <= 2^32 for every array.
+constexpr int ZM_RSRC_DW3 = 0x00020000;  // dword 3 of a raw buffer resource: 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t zrsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, ZM_RSRC_DW3);
+}
+template <int LPT>
+__device__ __forceinline__ Vec<LPT> bld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  Vec<LPT> v;
+  if constexpr (LPT == 2) {
+    const double2 t = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    v.v[0] = t.x;
+    v.v[1] = t.y;
+  } else {
+    v.v[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+  return v;
+}
+template <int LPT>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, const Vec<LPT>& v) {
+  if constexpr (LPT == 2) {
+    using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, make_double2(v.v[0], v.v[1])), r, off, 0, 0);
+  } else {
+    using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v.v[0]), r, off, 0, 0);
+  }
+}
+
 struct ZArgs {
   ApplyArgs a;
   const int* __restrict__ mcube;     // march cube: site -> point index or -1
@@ -58,16 +91,10 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   double* const lds1 = zsh + ZS * LC;                    // plane buffer 1
   int* const tidx = reinterpret_cast<int*>(zsh + 2 * ZS * LC);  // item index table
   const ApplyArgs& A = za.a;
-  const double* const a_fi = A.fi;
-  double* const a_fo = A.fo;
-  const double* const a_zrow = A.zrow;
-  const int a_np = A.np, a_lim = A.lim;
+  const int a_np = A.np;
   const double* const a_v = A.v;
   const int* const a_kslot = A.kslot;
-  const double* const a_ksum = A.ksum;
-  double* const a_yout = A.yout;
   double* const a_acc = A.acc;
-  const double* const a_acc_in = A.acc_in ? A.acc_in : A.acc;
   const double a_c1r = A.c1r, a_c1i = A.c1i, a_c2r = A.c2r, a_c2i = A.c2i;
   const int a_has_c1 = A.has_c1;
   double a_w[25];
@@ -80,8 +107,20 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const int x0 = it.x * ZT + ZR, y0 = it.y * ZT + ZR;
   const long mxl = za.mx;
   const long sxy = mxl * za.my;
-  const size_t L = (size_t)A.L;
   const int* __restrict__ mc = za.mcube;
+  const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
+  // row q of an array starts at byte q << sh; arrays without a role here get no records
+  const int sh = __builtin_ctz((unsigned)A.L) + 3;
+  const uint32_t lbyte = (uint32_t)l * 8u;
+  const uint32_t np_bytes = (uint32_t)a_np << sh;
+  const auto r_fi = zrsrc(A.fi, (uint32_t)A.lim << sh);
+  const auto r_v = zrsrc(a_v, a_v ? (uint32_t)a_np * 8u : 0u);
+  const auto r_k = zrsrc(A.ksum, NL ? (uint32_t)A.nk << sh : 0u);
+  const auto r_ai = zrsrc(A.acc_in ? A.acc_in : a_acc, has_acc ? np_bytes : 0u);
+  const auto r_fo = zrsrc(A.fo, EPI == EPI_STORE ? np_bytes : 0u);
+  const auto r_y = zrsrc(A.yout, A.yout ? np_bytes : 0u);
+  const auto r_acc = zrsrc(a_acc, has_acc ? np_bytes : 0u);
+  auto roff = [&](int q) { return ((uint32_t)q << sh) + lbyte; };
   // two halo units per thread: h = col + 64*i, same lane pair as the thread
   int hlds[2];
 #pragma unroll
@@ -124,11 +163,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
     return slot * LC + kp * LPT;
   };
 
-  // value of point q (zero row for boundary/non-points when q >= lim): branch-free load
-  auto val = [&](int q) {
-    const double* src = ((unsigned)q < (unsigned)a_lim) ? a_fi + (size_t)q * L + l : a_zrow;
-    return ldv<LPT>(src);
-  };
+  // value of point q (zero for boundary/non-points when q >= lim or q = -1)
+  auto val = [&](int q) { return bld<LPT>(r_fi, roff(q)); };
   auto lds_st = [&](int buf, int off, const Vec<LPT>& v) {
     double* b = buf ? lds1 : lds0;
     stv<LPT>(b + off, v);
@@ -218,21 +254,20 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   double vr[AR] = {};
   Vec<LPT> kr[AR];
   Vec<LPT> ar[AR];
-  const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
-  // Every global load below is issued unconditionally (out-of-range requests read the
-  // zero row): with loads behind branches the compiler cannot count the loads still in
-  // flight and drains the whole prefetch queue at every plane.
+  // Every global load below is issued unconditionally (out-of-range requests read zeros):
+  // with loads behind branches the compiler cannot count the loads still in flight and
+  // drains the whole prefetch queue at every plane.
   // per-point data of plane P (< ze + 4) for its output point: potential, ket sum row,
   // accumulator row
   auto load_aux = [&](int slot, int P) {
     const int q = own_idx(P);
-    const bool ok = P < ze && (unsigned)q < (unsigned)a_np;
-    vr[slot] = *((a_v && ok) ? a_v + q : a_zrow);
+    const int qo = (P < ze && (unsigned)q < (unsigned)a_np) ? q : -1;
+    vr[slot] = bld<1>(r_v, (uint32_t)qo << 3).v[0];
     if constexpr (NL) {
       const int ks = tks[(P - zb + 4) * 64 + col];
-      kr[slot] = ldv<LPT>((ok && ks >= 0) ? a_ksum + (size_t)ks * L + l : a_zrow);
+      kr[slot] = bld<LPT>(r_k, roff(qo >= 0 ? ks : -1));
     }
-    if constexpr (EPI == EPI_TAYLOR) ar[slot] = ldv<LPT>((has_acc && ok) ? a_acc_in + (size_t)q * L + l : a_zrow);
+    if constexpr (EPI == EPI_TAYLOR) ar[slot] = bld<LPT>(r_ai, roff(qo));
   };
 #pragma unroll
   for (int k = 0; k < 10; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
@@ -292,9 +327,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
           for (int k = 0; k < LPT; ++k) s.v[k] = s.v[k] + kr[kk % AR].v[k];
         }
         if constexpr (EPI == EPI_STORE) {
-          stv<LPT>(a_fo + (size_t)q0 * L + l, s);
+          bst<LPT>(r_fo, roff(q0), s);
         } else {
-          if (a_yout) stv<LPT>(a_yout + (size_t)q0 * L + l, s);
+          bst<LPT>(r_y, roff(q0), s);  // dropped when there is no yout
           if (has_acc) {
             Vec<LPT> av = ar[kk % AR];
             if constexpr (LPT == 2) {
@@ -310,7 +345,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
               if (a_has_c1) av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
               av.v[0] = av.v[0] + zmul_lane(a_c2r, a_c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
             }
-            stv<LPT>(a_acc + (size_t)q0 * L + l, av);
+            bst<LPT>(r_acc, roff(q0), av);
           }
         }
       }
