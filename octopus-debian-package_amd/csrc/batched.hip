@@ -43,6 +43,8 @@ struct Pair {
   double* ksum = nullptr;
   double* y[2] = {nullptr, nullptr};
   const int4* items = nullptr;   // the operator's z-march items for this pair's segment length
+  const int* tab = nullptr;      // and their index tables
+  const int* tab_off = nullptr;
   int nitems = 0;
 };
 
@@ -165,14 +167,13 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
         z.a.c2i = b->zf[k].imag();
       }
     }
-    z.mcube = op->d_mcube;
-    z.mx = op->mx;
-    z.my = op->my;
     z.items = p.items;
+    z.tab = p.tab;
+    z.tab_off = p.tab_off;
     z.sched = nullptr;
     // every pointer the kernel dereferences must exist
     // (the H-apply descriptor may lack fo: octhip_batched_apply refuses to run without hpsi)
-    const bool ok = z.a.fi && z.a.zrow && z.mcube && z.items &&
+    const bool ok = z.a.fi && z.items && z.tab && z.tab_off &&
                     (!z.a.kslot || z.a.ksum) && (k < 2 || z.a.acc) && (k == 0 || k == b->order || z.a.yout);
     if (!ok) return fail(OCTHIP_ERR_STATE, "batched: incomplete descriptor for pair %zu, pass %d", i, k);
     if (!zmarch_fits(op, z.a))
@@ -251,6 +252,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc));
     if (!rc) {
       p.items = h->kin->d_items;
+      p.tab = h->kin->d_tab;
+      p.tab_off = h->kin->d_tab_off;
       p.nitems = h->kin->nitems;
       b->zseg = std::max(b->zseg, h->kin->zseg_built);
     }

@@ -147,7 +147,9 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
       op->tile_zhi[t] = std::max(op->tile_zhi[t], (int)z);
     }
   }
-  return upload(&op->d_mcube, mc.data(), mc.size(), s);
+  (void)s;
+  op->mcube = std::move(mc);
+  return OCTHIP_SUCCESS;
 }
 
 }  // namespace
@@ -182,6 +184,8 @@ int zmarch_items(octhip_nl_operator_s* op, int zs) {
   for (auto& pl : op->plans)
     if (pl.zseg == zs) {
       op->d_items = pl.d_items;
+      op->d_tab = pl.d_tab;
+      op->d_tab_off = pl.d_tab_off;
       op->items = pl.items;
       op->nitems = (int)pl.items.size();
       op->sched = pl.sched;
@@ -204,11 +208,49 @@ int zmarch_items(octhip_nl_operator_s* op, int zs) {
         const int ze = std::min(zb + zs, op->tile_zhi[t] + 1);
         items.push_back(make_int4(tx, ty, zb, ze));
       }
+  // Index table of every item, read by the kernel's prologue as one coalesced block: for
+  // planes P = zb-4 .. ze+3, 192 sites each (the 64 tile columns, then the 128 halo sites
+  // of the cross-shaped window in the kernel's slot order), the point index at the site
+  // or -1; halo entries only for the item's own planes zb .. ze-1.
+  std::vector<int> tab, toff(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    const int4 it = items[i];
+    toff[i] = (int)tab.size();
+    const long x0 = (long)it.x * ZT + ZR, y0 = (long)it.y * ZT + ZR;
+    for (int P = it.z - 4; P < it.w + 4; ++P)
+      for (int t = 0; t < 192; ++t) {
+        long hx, hy;
+        if (t < 64) {
+          hx = x0 + (t & 7);
+          hy = y0 + (t >> 3);
+        } else {
+          const int h = t - 64;
+          if (h < 32) { hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); }
+          else if (h < 64) { hx = x0 + 8 + ((h - 32) & 3); hy = y0 + ((h - 32) >> 2); }
+          else if (h < 96) { hx = x0 + ((h - 64) & 7); hy = y0 - 4 + ((h - 64) >> 3); }
+          else { hx = x0 + ((h - 96) & 7); hy = y0 + 8 + ((h - 96) >> 3); }
+        }
+        const bool need = t < 64 || (P >= it.z && P < it.w);
+        tab.push_back(need ? op->mcube[(size_t)hx + (size_t)op->mx * ((size_t)hy + (size_t)op->my * P)] : -1);
+      }
+  }
+  if (tab.size() > (size_t)INT_MAX) return fail(OCTHIP_ERR_UNSUPPORTED, "z-march index tables too large");
   int4* d = nullptr;
+  int* dt = nullptr;
+  int* dto = nullptr;
   int rc = upload(&d, items.data(), items.size(), nullptr);
-  if (rc) return rc;
-  op->plans.push_back({zs, d, items, {}, {}});
+  if (!rc) rc = upload(&dt, tab.data(), tab.size(), nullptr);
+  if (!rc) rc = upload(&dto, toff.data(), toff.size(), nullptr);
+  if (rc) {
+    dfree(d);
+    dfree(dt);
+    dfree(dto);
+    return rc;
+  }
+  op->plans.push_back({zs, d, dt, dto, items, {}, {}});
   op->d_items = d;
+  op->d_tab = dt;
+  op->d_tab_off = dto;
   op->items = items;
   op->nitems = (int)items.size();
   op->sched.clear();
@@ -299,9 +341,8 @@ int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t
   if (rc) return rc;
   ZArgs z;
   z.a = a;
-  z.mcube = op->d_mcube;
-  z.mx = op->mx;
-  z.my = op->my;
+  z.tab = op->d_tab;
+  z.tab_off = op->d_tab_off;
   z.items = op->d_items;
   z.sched = sched;
   return launch_zmarch(z, lc, lpt, epi, len, op->zseg_built, s);
@@ -382,11 +423,12 @@ extern "C" int octhip_nl_operator_destroy(octhip_nl_operator_t op) {
   dfree(op->d_cidx);
   dfree(op->d_cube);
   dfree(op->d_dcube);
-  dfree(op->d_mcube);
   for (auto& pl : op->plans)
     if (pl.zseg == op->zseg_built) pl.sched = op->sched;
   for (auto& pl : op->plans) {
     dfree(pl.d_items);
+    dfree(pl.d_tab);
+    dfree(pl.d_tab_off);
     for (auto& e : pl.sched) dfree(e.second);
   }
   for (void* p : op->retired) dfree(p);

@@ -103,7 +103,7 @@ struct octhip_nl_operator_s {
   // z-march (order-4 star on a lattice): padded march cube and (tile, z-segment) items
   int kernel = 0;                        // 0 = auto, 1 = gather, 2 = z-march
   bool zmarch_ok = false;
-  int* d_mcube = nullptr;
+  std::vector<int> mcube;                // site -> point index or -1 (host; builds the item tables)
   int mx = 0, my = 0, mz = 0;
   int zseg = 0;                          // output planes per work item, 0 = chosen per launch
   int lane_chunk = 0;                    // lanes per workgroup (0 = automatic)
@@ -112,6 +112,8 @@ struct octhip_nl_operator_s {
   std::vector<int> tile_zlo, tile_zhi;   // per tile: inner plane range (cube z), -1 if empty
   int ntx = 0, nty = 0;
   int4* d_items = nullptr;
+  int* d_tab = nullptr;                  // per item: site -> point index table (planes zb-4 .. ze+3)
+  int* d_tab_off = nullptr;              // per item: first int of its table
   int nitems = 0;
   std::vector<int4> items;
   std::vector<std::pair<int, int2*>> sched;  // lane chunks -> device schedule
@@ -121,6 +123,8 @@ struct octhip_nl_operator_s {
   struct ZPlan {
     int zseg;
     int4* d_items;
+    int* d_tab;
+    int* d_tab_off;
     std::vector<int4> items;
     std::vector<std::pair<int, int2*>> sched;
     std::vector<int> sched_len;

@@ -72,9 +72,9 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, cons
 
 struct ZArgs {
   ApplyArgs a;
-  const int* __restrict__ mcube;     // march cube: site -> point index or -1
-  int mx, my;                        // x, y extent of the march cube
   const int4* __restrict__ items;    // {tile x, tile y, first plane, end plane} (cube z)
+  const int* __restrict__ tab;       // per item: index table of planes zb-4 .. ze+3, 192 sites each
+  const int* __restrict__ tab_off;   // per item: first int of its table (a multiple of 4)
   const int2* __restrict__ sched;    // blockIdx -> {item, lane chunk}
 };
 
@@ -82,7 +82,7 @@ struct ZArgs {
 // za may live in kernel arguments (single operator) or in global memory (one descriptor
 // per system of a batched launch): every field is copied to a local once.
 template <int LPT, int LC, int EPI, bool NL>
-__device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, const int chunk) {
+__device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, const int item, const int chunk) {
   constexpr int PAIRS = LC / LPT;    // threads per column
   constexpr int NT = 64 * PAIRS;
   extern __shared__ double2 zsh2[];  // 16-byte aligned: row chunks move as ds_*_b128
@@ -104,10 +104,6 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const int col = threadIdx.x / PAIRS;
   const int cx = col & 7, cy = col >> 3;
   const int l = chunk * LC + kp * LPT;         // first lane of this thread
-  const int x0 = it.x * ZT + ZR, y0 = it.y * ZT + ZR;
-  const long mxl = za.mx;
-  const long sxy = mxl * za.my;
-  const int* __restrict__ mc = za.mcube;
   const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
   // row q of an array starts at byte q << sh; arrays without a role here get no records
   const int sh = __builtin_ctz((unsigned)A.L) + 3;
@@ -121,31 +117,11 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const auto r_y = zrsrc(A.yout, A.yout ? np_bytes : 0u);
   const auto r_acc = zrsrc(a_acc, has_acc ? np_bytes : 0u);
   auto roff = [&](int q) { return ((uint32_t)q << sh) + lbyte; };
-  // two halo units per thread: h = col + 64*i, same lane pair as the thread
+  // two halo units per thread: site slot 64 + col + 64*i, same lane pair as the thread
   int hlds[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int h = col + 64 * i;
-    int hx, hy, lx, ly;
-    if (h < 32) {
-      hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); lx = h & 3; ly = 4 + (h >> 2);
-    } else if (h < 64) {
-      const int g = h - 32;
-      hx = x0 + 8 + (g & 3); hy = y0 + (g >> 2); lx = 12 + (g & 3); ly = 4 + (g >> 2);
-    } else if (h < 96) {
-      const int g = h - 64;
-      hx = x0 + (g & 7); hy = y0 - 4 + (g >> 3); lx = 4 + (g & 7); ly = g >> 3;
-    } else {
-      const int g = h - 96;
-      hx = x0 + (g & 7); hy = y0 + 8 + (g >> 3); lx = 4 + (g & 7); ly = 12 + (g >> 3);
-    }
-    (void)hx;
-    (void)hy;
-    (void)lx;
-    (void)ly;
-    hlds[i] = (64 + h) * LC + kp * LPT;
-  }
-  // Cross-shaped window in LDS, one LC-lane chunk per site slot:
+  for (int i = 0; i < 2; ++i) hlds[i] = (64 + col + 64 * i) * LC + kp * LPT;
+  // Cross-shaped window in LDS, one LC-lane chunk per site slot ((x0, y0) = first tile column):
   //   slot c (0..63)          tile column c = cx + 8 cy
   //   slot 64 + 4 r + k       left strip,  x = x0-4+k, y = y0+r   (k < 4, r < 8)
   //   slot 96 + 4 r + k       right strip, x = x0+8+k, y = y0+r
@@ -178,37 +154,23 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   int* const tks = tidx + (ze - zb + 8) * 192;  // ket slots of the own sites
   // ---- index table of the item in LDS: 192 sites per plane (64 own columns, then the
   //      128 halo sites), planes zb-4 .. ze+3; halo entries exist for planes zb .. ze-1.
-  //      Index reads then cost LDS, not the in-order global-load counter.
+  //      Precomputed per item, so the prologue is one coalesced block copy; index reads
+  //      then cost LDS, not the in-order global-load counter.
   {
-    const int total = (ze - zb + 8) * 192;
-    for (int base = 0; base < total; base += NT * 8) {
-      int tq[8];
+    const int n4 = (ze - zb + 8) * 192 / 4;
+    const int4* __restrict__ g4 = reinterpret_cast<const int4*>(za.tab + za.tab_off[item]);
+    int4* const t4 = reinterpret_cast<int4*>(tidx);
+    for (int base = 0; base < n4; base += NT * 8) {
+      int4 tq[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = base + threadIdx.x + u * NT;
-        tq[u] = -1;
-        if (e < total) {
-          const int r = e / 192, t = e - r * 192;
-          const int P = zb - 4 + r;
-          long xy;
-          if (t < 64) {
-            xy = (x0 + (t & 7)) + mxl * (y0 + (t >> 3));
-          } else {
-            const int h = t - 64;
-            int hx, hy;
-            if (h < 32) { hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); }
-            else if (h < 64) { hx = x0 + 8 + ((h - 32) & 3); hy = y0 + ((h - 32) >> 2); }
-            else if (h < 96) { hx = x0 + ((h - 64) & 7); hy = y0 - 4 + ((h - 64) >> 3); }
-            else { hx = x0 + ((h - 96) & 7); hy = y0 + 8 + ((h - 96) >> 3); }
-            xy = hx + mxl * hy;
-          }
-          if (t < 64 || (P >= zb && P < ze)) tq[u] = mc[xy + sxy * P];
-        }
+        tq[u] = g4[e < n4 ? e : n4 - 1];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = base + threadIdx.x + u * NT;
-        if (e < total) tidx[e] = tq[u];
+        if (e < n4) t4[e] = tq[u];
       }
     }
     if constexpr (NL) {
@@ -357,7 +319,7 @@ template <int LPT, int LC, int EPI, bool NL>
 __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
   const int2 sc = za.sched[blockIdx.x];
   if (sc.x < 0) return;              // padding entry of the schedule (whole workgroup)
-  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.x], sc.y);
+  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.x], sc.x, sc.y);
 }
 
 // Several operators (systems) and batches in one launch: sched[b] = {descriptor, item,
@@ -368,7 +330,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch_multi(const ZArgs* __r
   const int4 sc = sched[blockIdx.x];
   if (sc.x < 0) return;
   const ZArgs& za = zs[sc.x];
-  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.y], sc.z);
+  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.y], sc.y, sc.z);
 }
 
 // dynamic LDS bytes of k_zmarch for a lane chunk and segment length
