@@ -173,29 +173,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
         if (e < n4) t4[e] = tq[u];
       }
     }
-    if constexpr (NL) {
-      __syncthreads();
-      const int totk = (ze - zb + 8) * 64;
-      for (int base = 0; base < totk; base += NT * 8) {
-        int kq[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = base + threadIdx.x + u * NT;
-          kq[u] = -1;
-          if (e < totk) {
-            const int q = tidx[(e >> 6) * 192 + (e & 63)];
-            if ((unsigned)q < (unsigned)a_np) kq[u] = a_kslot[q];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = base + threadIdx.x + u * NT;
-          if (e < totk) tks[e] = kq[u];
-        }
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();
   auto own_idx = [&](int P) { return tidx[(P - zb + 4) * 192 + col]; };
   auto halo_idx = [&](int P, int i) { return tidx[(P - zb + 4) * 192 + 64 + col + 64 * i]; };
 
@@ -237,6 +216,28 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   for (int k = 0; k < HD; ++k)
 #pragma unroll
     for (int i = 0; i < 2; ++i) hv[k][i] = val(zb + k < ze ? halo_idx(zb + k, i) : -1);
+  // ket slots of the own sites, looked up while the first planes' values are in flight
+  // (one unconditional range-checked load per entry; -1 where the site has no point)
+  if constexpr (NL) {
+    const auto r_ks = zrsrc(a_kslot, (uint32_t)a_np * 4u);
+    const int totk = (ze - zb + 8) * 64;
+    for (int base = 0; base < totk; base += NT * 8) {
+      int kq[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = min(base + (int)threadIdx.x + u * NT, totk - 1);
+        const int q = tidx[(e >> 6) * 192 + (e & 63)];
+        const int k = (int)__builtin_amdgcn_raw_buffer_load_b32(r_ks, (uint32_t)q << 2, 0, 0);
+        kq[u] = (unsigned)q < (unsigned)a_np ? k : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = base + threadIdx.x + u * NT;
+        if (e < totk) tks[e] = kq[u];
+      }
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int k = 0; k < AD; ++k) load_aux(k, zb + k);
 
