@@ -411,7 +411,7 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="hamiltonian", choices=["hamiltonian", "laplacian", "taylor", "adjoint", "batched"])
     ap.add_argument("--kernel", type=int, default=None, help="0 auto, 1 gather, 2 z-march")
     ap.add_argument("--xcd-chunk", type=int, default=None)

@@ -107,7 +107,7 @@ extern "C" int octhip_hamiltonian_autotune(octhip_hamiltonian_t h, int ldf, int 
   if (op->zmarch_ok && op->n == 25) {
     for (int lc : {8, 16, 32})
       for (int lpt : {1, 2})
-        for (int zs : {8, 16, 32}) {
+        for (int zs : {8, 16, 32, 48}) {
           if (lc > L || lpt > lc || (lc == 32 && lpt == 1)) continue;
           cands.push_back(TuneVal{2, lc, lpt, zs, op->tile_iters, bra0, 0.0});
         }
