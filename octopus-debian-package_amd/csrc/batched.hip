@@ -23,7 +23,7 @@
 namespace octhip {
 void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a);
 int zmarch_items(octhip_nl_operator_s* op, int zs);
-int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk);
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident);
 bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a);
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
                         hipStream_t s);
@@ -249,7 +249,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     }
     if (!rc) rc = dalloc((void**)&p.y[0], sizeof(double) * h->np * L);
     if (!rc) rc = dalloc((void**)&p.y[1], sizeof(double) * h->np * L);
-    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc));
+    // one resident workgroup per CU: the batched kernels take the longer segments
+    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, 1));
     if (!rc) {
       p.items = h->kin->d_items;
       p.tab = h->kin->d_tab;

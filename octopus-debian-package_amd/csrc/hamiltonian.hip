@@ -12,7 +12,7 @@ void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a);
 int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s, double* zf = nullptr,
                     size_t zfirst = 0, size_t zcount = 0);
 int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t s);
-int prepare_operator(octhip_nl_operator_s* op, int L);
+int prepare_operator(octhip_nl_operator_s* op, int L, bool nl);
 }  // namespace octhip
 
 namespace {
@@ -428,7 +428,7 @@ void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
   int rc = ensure_work(h, ldf, s);
   if (!rc) rc = ensure_taylor(h, ldf, s);
-  if (!rc) rc = prepare_operator(h->kin, 1 << ldf);
+  if (!rc) rc = prepare_operator(h->kin, 1 << ldf, h->nmat > 0 && h->nslot > 0);
   return rc;
 }
 

@@ -155,20 +155,72 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
 }  // namespace
 
 namespace octhip {
-// Segment length of a launch with nchunk lane chunks: the operator's setting, or the
-// longest of 48, 32, 24, 16 planes that still gives the launch 1024 workgroups, else 8
-// (measured on MI355X with equal-length segments: 48 is 5% faster than 32 on a 97-plane
-// sphere once there are a few workgroups per CU; short segments cost overlap reads but
-// keep small meshes from leaving CUs idle).
-int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk) {
+int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl);
+
+// Segment length of a launch with nchunk lane chunks of lc lanes: the operator's setting,
+// or the candidate (48, 32, 24, 20, 16, 12, 8 planes) with the smallest estimated time,
+// in plane steps of one workgroup:
+//   work  = nchunk * sum over items (planes + 8 overlap planes + ZM_PROLOGUE)
+//   time  = max(work / (CUs * g), longest item * r / g)
+// where r = workgroups resident per CU (2 when the instance's registers allow it and the
+// segment's LDS fits twice, else 1) and g = the throughput of r resident workgroups
+// relative to one (ZM_TWO_GAIN for two).  The first term is the whole launch spread over
+// the chip, the second its critical path: long segments cost little overlap but leave
+// CUs idle on small meshes.  Measured on MI355X: R=12 sphere, 32 complex states: 24 planes
+// with two resident (H-apply) and 48 with one (Taylor); the 64^3 Laplacian with 8 lanes:
+// 8 planes (24 us against 38 us at 32).
+constexpr double ZM_PROLOGUE = 4.0;  // item prologue, in plane steps
+constexpr double ZM_TWO_GAIN = 1.3;  // throughput of two resident workgroups against one
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident) {
   if (op->zseg > 0) return op->zseg;
-  for (int zs : {48, 32, 24, 16}) {
-    long n = 0;
-    for (size_t t = 0; t < op->tile_zlo.size(); ++t)
-      if (op->tile_zhi[t] >= 0) n += (op->tile_zhi[t] - op->tile_zlo[t] + zs) / zs;
-    if (n * nchunk >= 1024) return zs;
+  for (const auto& c : op->zseg_cache)
+    if (c.x == nchunk && c.y == lc && c.z == resident) return c.w;
+  static int ncu = 0;
+  if (ncu <= 0 && (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, current_device()) != hipSuccess ||
+                   ncu <= 0))
+    ncu = 256;
+  const bool two_ok = resident >= 2;
+  double best_t = 1e300;
+  int best = 8;
+  for (int zs : {48, 32, 24, 20, 16, 12, 8}) {
+    const size_t shm = zmarch_shmem(lc, zs);
+    if (shm > 160 * 1024) continue;
+    const int r = (two_ok && shm <= 80 * 1024) ? 2 : 1;
+    const double g = r == 2 ? ZM_TWO_GAIN : 1.0;
+    double work = 0.0, cmax = 0.0;
+    for (size_t t = 0; t < op->tile_zlo.size(); ++t) {
+      if (op->tile_zhi[t] < 0) continue;
+      const int len = op->tile_zhi[t] - op->tile_zlo[t] + 1;
+      const int n = (len + zs - 1) / zs;
+      work += len + n * (8.0 + ZM_PROLOGUE);
+      cmax = std::max(cmax, (len + n - 1) / n + 8.0 + ZM_PROLOGUE);
+    }
+    const double tm = std::max(nchunk * work / (ncu * g), cmax * r / g);
+    if (tm < best_t) {
+      best_t = tm;
+      best = zs;
+    }
   }
-  return 8;
+  op->zseg_cache.push_back(make_int4(nchunk, lc, resident, best));
+  return best;
+}
+
+// lane chunk and lanes per thread of a z-march launch over rows of L doubles
+void zmarch_shape(const octhip_nl_operator_s* op, int L, int* lc, int* lpt) {
+  int c = op->lane_chunk > 0 ? op->lane_chunk : 16;
+  if (c > L) c = L;
+  int t = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
+  if (t > c) t = c;
+  if (c == 32) t = 2;  // 1024 threads per workgroup at most
+  *lc = c;
+  *lpt = t;
+}
+
+// segment length of a launch of this epilogue (with or without ket rows)
+int zmarch_zseg_for(const octhip_nl_operator_s* op, int L, int epi, bool nl) {
+  int lc, lpt;
+  zmarch_shape(op, L, &lc, &lpt);
+  return zmarch_zseg(op, L / lc, lc, zmarch_reg_blocks(lc, lpt, epi, nl));
 }
 
 // (tile, z-segment) work items for segment length zs; every list built stays allocated
@@ -309,18 +361,24 @@ bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a) {
   return (uint64_t)(op->np_part + 1) * rowb <= lim && (!a.kslot || (uint64_t)(a.nk + 1) * rowb <= lim);
 }
 
-// Build every lazily created device structure an apply with rows of L doubles needs, so
-// that later applies allocate nothing (required before a graph capture).
-int prepare_operator(octhip_nl_operator_s* op, int L) {
+// Build every lazily created device structure an apply with rows of L doubles needs (both
+// epilogues; nl = with ket rows), so that later applies allocate nothing (required before
+// a graph capture).
+int prepare_operator(octhip_nl_operator_s* op, int L, bool nl) {
   const bool zm = op->zmarch_ok && op->kernel != 1 && op->n == 25;
   if (!zm) return OCTHIP_SUCCESS;
-  int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
-  if (lc > L) lc = L;
-  int rc = zmarch_items(op, zmarch_zseg(op, L / lc));
-  if (rc) return rc;
-  const int2* sched = nullptr;
-  int len = 0;
-  return zmarch_sched(op, L / lc, &sched, &len);
+  int lc, lpt;
+  zmarch_shape(op, L, &lc, &lpt);
+  for (int epi : {EPI_STORE, EPI_TAYLOR}) {
+    if (epi == EPI_TAYLOR && lc < 2) continue;
+    int rc = zmarch_items(op, zmarch_zseg_for(op, L, epi, nl));
+    if (rc) return rc;
+    const int2* sched = nullptr;
+    int len = 0;
+    rc = zmarch_sched(op, L / lc, &sched, &len);
+    if (rc) return rc;
+  }
+  return OCTHIP_SUCCESS;
 }
 
 // Apply through the best kernel for this operator: z-march for the order-4 star on a
@@ -330,14 +388,11 @@ int launch_operator(octhip_nl_operator_s* op, ApplyArgs& a, int epi, hipStream_t
     return fail(OCTHIP_ERR_STATE, "apply: incomplete kernel arguments");
   const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25 && zmarch_fits(op, a);
   if (!zm) return launch_apply(a, op->lattice && a.n > 0, epi, s);
-  int lc = op->lane_chunk > 0 ? op->lane_chunk : 16;
-  if (lc > a.L) lc = a.L;
-  int rc = zmarch_items(op, zmarch_zseg(op, a.L / lc));
+  int lc, lpt;
+  zmarch_shape(op, a.L, &lc, &lpt);
+  int rc = zmarch_items(op, zmarch_zseg_for(op, a.L, epi, a.kslot != nullptr));
   if (rc) return rc;
   if (op->nitems == 0 || a.np <= 0) return OCTHIP_SUCCESS;
-  int lpt = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
-  if (lpt > lc) lpt = lc;
-  if (lc == 32) lpt = 2;  // 1024 threads per workgroup at most
   if (zmarch_shmem(lc, op->zseg_built) > 160 * 1024)
     return fail(OCTHIP_ERR_INVALID, "z-march: lane chunk %d with z segment %d needs more than 160 KB of LDS", lc,
                 op->zseg_built);

@@ -1,11 +1,12 @@
 // Autotuner of the H-apply launch shape.
 //
 // For one Hamiltonian and one row length (ldf) it times every candidate launch shape of
-// the fused stencil pass (z-march lane chunk, lanes per thread and z segment; the tiled
-// gather with its tile depth) and both forms of the projector bra (FP64 MFMA, plain) on
-// scratch batches, with HIP events on the caller's stream, and leaves the fastest set on
-// the handle.  Results are kept in a process-wide table keyed by the problem shape
-// (np, np_part, projector matrices, ldf), so a second Hamiltonian of the same shape is
+// the fused stencil pass (z-march lane chunk and lanes per thread, the z segment left to
+// the per-launch choice; the tiled gather with its tile depth) and both forms of the
+// projector bra (FP64 MFMA, plain) on scratch batches, with HIP events on the caller's
+// stream, and leaves the fastest set on the handle.  Results are kept in a process-wide
+// table keyed by the problem shape (np, np_part, projector matrices, ldf), so a second
+// Hamiltonian of the same shape is
 // tuned without timing; the table is saved to / loaded from a text file.  Every candidate
 // computes the same result (the per-lane arithmetic does not depend on the launch shape,
 // except the bra's summation order between MFMA and plain, both within the H-apply
@@ -101,16 +102,18 @@ extern "C" int octhip_hamiltonian_autotune(octhip_hamiltonian_t h, int ldf, int 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "autotune fill: %s", hipGetErrorString(e));
   }
-  // candidates of the stencil pass (bra fixed to MFMA where it applies), then the bra
+  // candidates of the stencil pass (bra fixed to MFMA where it applies), then the bra.
+  // The z segment stays automatic (0): its choice depends on the epilogue (the Taylor
+  // instance keeps one workgroup per CU, the H-apply instance two), and this timing sees
+  // only the H-apply.
   std::vector<TuneVal> cands;
   const int bra0 = L >= 16 ? 1 : 0;
   if (op->zmarch_ok && op->n == 25) {
     for (int lc : {8, 16, 32})
-      for (int lpt : {1, 2})
-        for (int zs : {8, 16, 32, 48}) {
-          if (lc > L || lpt > lc || (lc == 32 && lpt == 1)) continue;
-          cands.push_back(TuneVal{2, lc, lpt, zs, op->tile_iters, bra0, 0.0});
-        }
+      for (int lpt : {1, 2}) {
+        if (lc > L || lpt > lc || (lc == 32 && lpt == 1)) continue;
+        cands.push_back(TuneVal{2, lc, lpt, 0, op->tile_iters, bra0, 0.0});
+      }
   }
   for (int gt : {2, 4}) cands.push_back(TuneVal{1, 0, 0, op->zseg, gt, bra0, 0.0});
   TuneVal best{};

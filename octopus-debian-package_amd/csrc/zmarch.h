@@ -34,7 +34,8 @@ constexpr int ZPADZ = 8;             // march-cube padding below/above the inner
 #define ZM_HALO_AHEAD 2                // planes of halo values loaded ahead (2..4 measured within 2%)
 #endif
 #ifndef ZM_AUX_AHEAD
-#define ZM_AUX_AHEAD 4                 // planes of output-point data loaded ahead
+#define ZM_AUX_AHEAD 2                 // planes of output-point data loaded ahead (2 keeps the
+                                       // H-apply instance at 126 VGPRs: two workgroups per CU)
 #endif
 
 // Raw buffer resources over the row arrays (gfx9 range checking, stride 0): a load whose
