@@ -24,6 +24,7 @@ namespace octhip {
 void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a);
 int zmarch_items(octhip_nl_operator_s* op, int zs);
 int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident);
+int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a);
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
                         hipStream_t s);
@@ -232,6 +233,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   b->mfma = minL >= 16;
   for (int i = 0; i < npairs; ++i) b->mfma = b->mfma && hms[i]->use_mfma;
   taylor_coefs(deltat, 0, order, b->zf);
+  const int resident = std::min(zmarch_reg_blocks(b->lc, b->lpt, EPI_STORE, b->nl, true),
+                                zmarch_reg_blocks(b->lc, b->lpt, EPI_TAYLOR, b->nl, true));
   int rc = OCTHIP_SUCCESS;
   // pairs and their work buffers
   for (int i = 0; i < npairs && !rc; ++i) {
@@ -249,8 +252,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     }
     if (!rc) rc = dalloc((void**)&p.y[0], sizeof(double) * h->np * L);
     if (!rc) rc = dalloc((void**)&p.y[1], sizeof(double) * h->np * L);
-    // one resident workgroup per CU: the batched kernels take the longer segments
-    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, 1));
+    // the items serve both epilogues: segments for the residency both instances allow
+    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident));
     if (!rc) {
       p.items = h->kin->d_items;
       p.tab = h->kin->d_tab;

@@ -155,7 +155,7 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
 }  // namespace
 
 namespace octhip {
-int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl);
+int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 
 // Segment length of a launch with nchunk lane chunks of lc lanes: the operator's setting,
 // or the candidate (48, 32, 24, 20, 16, 12, 8 planes) with the smallest estimated time,
@@ -167,7 +167,7 @@ int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl);
 // relative to one (ZM_TWO_GAIN for two).  The first term is the whole launch spread over
 // the chip, the second its critical path: long segments cost little overlap but leave
 // CUs idle on small meshes.  Measured on MI355X: R=12 sphere, 32 complex states: 24 planes
-// with two resident (H-apply) and 48 with one (Taylor); the 64^3 Laplacian with 8 lanes:
+// with two resident workgroups beat 48 with one by 7%; the 64^3 Laplacian with 8 lanes:
 // 8 planes (24 us against 38 us at 32).
 constexpr double ZM_PROLOGUE = 4.0;  // item prologue, in plane steps
 constexpr double ZM_TWO_GAIN = 1.3;  // throughput of two resident workgroups against one
@@ -220,7 +220,7 @@ void zmarch_shape(const octhip_nl_operator_s* op, int L, int* lc, int* lpt) {
 int zmarch_zseg_for(const octhip_nl_operator_s* op, int L, int epi, bool nl) {
   int lc, lpt;
   zmarch_shape(op, L, &lc, &lpt);
-  return zmarch_zseg(op, L / lc, lc, zmarch_reg_blocks(lc, lpt, epi, nl));
+  return zmarch_zseg(op, L / lc, lc, zmarch_reg_blocks(lc, lpt, epi, nl, false));
 }
 
 // (tile, z-segment) work items for segment length zs; every list built stays allocated

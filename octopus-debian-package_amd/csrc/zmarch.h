@@ -9,8 +9,8 @@
 //     nothing;
 //   * the site -> point indices of the item (and the ket slots of its own sites) are
 //     staged in LDS once per item;
-//   * loads run ZM_HALO_AHEAD planes ahead (halo values), six ahead (own column values)
-//     and ZM_AUX_AHEAD ahead (potential, ket row, Taylor accumulator), into ring buffers
+//   * loads run a few planes ahead (own column values, halo values, and the potential,
+//     ket row and Taylor accumulator of the output point; depths in ZRings), into ring buffers
 //     with compile-time slots (the z loop is unrolled by the ring period); every load is
 //     unconditional, so the compiler's wait counts stay exact and a step waits only on
 //     loads issued planes earlier.
@@ -30,13 +30,21 @@ constexpr int ZR = 4;                // stencil reach
 constexpr int ZW = ZT + 2 * ZR;      // window side (tile + halo)
 constexpr int ZS = 192;              // sites of the cross-shaped plane window: 64 tile + 128 halo
 constexpr int ZPADZ = 8;             // march-cube padding below/above the inner points in z
-#ifndef ZM_HALO_AHEAD
-#define ZM_HALO_AHEAD 2                // planes of halo values loaded ahead (2..4 measured within 2%)
-#endif
-#ifndef ZM_AUX_AHEAD
-#define ZM_AUX_AHEAD 2                 // planes of output-point data loaded ahead (2 keeps the
-                                       // H-apply instance at 126 VGPRs: two workgroups per CU)
-#endif
+// Prefetch depths (planes loaded ahead) of the own column values, the halo values and the
+// output-point data, and the own-value ring length (= z loop unroll, >= own depth + 5).
+// The Taylor epilogue carries an accumulator row per plane more, so it takes shallow
+// rings (116 VGPRs: two 512-thread workgroups per CU, whose loads cover each other's
+// latency; 2.3% faster Taylor steps than with the deep rings at 143 VGPRs); the plain
+// store keeps the deep ones (126 VGPRs with ket rows, still two per CU; the shallow ones
+// cost the 8-lane Laplacian 12%).
+template <int EPI>
+struct ZRings {
+  static constexpr int own = 6, halo = 2, aux = 2, ring = 12;
+};
+template <>
+struct ZRings<EPI_TAYLOR> {
+  static constexpr int own = 5, halo = 1, aux = 1, ring = 10;
+};
 
 // Raw buffer resources over the row arrays (gfx9 range checking, stride 0): a load whose
 // byte offset lies at or beyond num_records returns zeros and such a store is dropped.
@@ -69,6 +77,14 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, cons
     using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v.v[0]), r, off, 0, 0);
   }
+}
+
+// smallest divisor of rq above d: the length of a prefetch ring d planes deep whose slots
+// stay compile-time constants in a z loop unrolled rq times
+constexpr int ring_above(int rq, int d) {
+  for (int r = d + 1; r < rq; ++r)
+    if (rq % r == 0) return r;
+  return rq;
 }
 
 struct ZArgs {
@@ -158,15 +174,18 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   //      Precomputed per item, so the prologue is one coalesced block copy; index reads
   //      then cost LDS, not the in-order global-load counter.
   {
+    // (through a buffer resource: the descriptors of a batched launch live in global
+    // memory, and a generic pointer loaded from there would be read with flat loads that
+    // the compiler must order against the LDS stores)
     const int n4 = (ze - zb + 8) * 192 / 4;
-    const int4* __restrict__ g4 = reinterpret_cast<const int4*>(za.tab + za.tab_off[item]);
+    const auto r_tab = zrsrc(za.tab + za.tab_off[item], (uint32_t)n4 * 16u);
     int4* const t4 = reinterpret_cast<int4*>(tidx);
     for (int base = 0; base < n4; base += NT * 8) {
       int4 tq[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = base + threadIdx.x + u * NT;
-        tq[u] = g4[e < n4 ? e : n4 - 1];
+        tq[u] = __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(r_tab, (uint32_t)e * 16u, 0, 0));
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -181,16 +200,16 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
 
   // Rings with static slots (the z loop is unrolled by RQ, so every slot index is a
   // compile-time constant and values stay in the registers their loads wrote):
-  //   ov[(P - zb + 4) % RQ]  own value of plane P,  P in z-4 .. z+6 (loaded 6 planes ahead)
+  //   ov[(P - zb + 4) % RQ]  own value of plane P,  P in z-4 .. z+OA (loaded OA planes ahead)
   //   hv[(P - zb) % HR]      halo values of plane P, P in z .. z+HD (loaded HD planes ahead)
   //   vr/kr/ar[(P - zb) % AR] output point data,     P in z .. z+AD (loaded AD planes ahead)
   // The load counter retires in order, so the shortest distance sets how many planes of
   // loads stay in flight: halo and point data go as deep as the own values allow.
-  constexpr int RQ = 12;
-  constexpr int HD = ZM_HALO_AHEAD, AD = ZM_AUX_AHEAD;
-  constexpr int HR = HD < 3 ? HD + 1 + (HD == 1) : 6;  // ring sizes dividing RQ
-  constexpr int AR = AD < 3 ? AD + 1 + (AD == 1) : 6;
-  static_assert(RQ % HR == 0 && RQ % AR == 0 && HR > HD && AR > AD && HD <= 6 && AD <= 6, "ring sizes");
+  constexpr int RQ = ZRings<EPI>::ring, OA = ZRings<EPI>::own;
+  constexpr int HD = ZRings<EPI>::halo, AD = ZRings<EPI>::aux;
+  constexpr int HR = ring_above(RQ, HD);  // ring sizes dividing RQ
+  constexpr int AR = ring_above(RQ, AD);
+  static_assert(OA >= 5 && RQ >= OA + 5 && HR > HD && AR > AD, "ring sizes");
   Vec<LPT> ov[RQ];
   Vec<LPT> hv[HR][2];
   double vr[AR] = {};
@@ -212,7 +231,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
     if constexpr (EPI == EPI_TAYLOR) ar[slot] = bld<LPT>(r_ai, roff(qo));
   };
 #pragma unroll
-  for (int k = 0; k < 10; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
+  for (int k = 0; k < OA + 4; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
 #pragma unroll
   for (int k = 0; k < HD; ++k)
 #pragma unroll
@@ -248,9 +267,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       const int z = z0 + kk;
       if (z >= ze) break;
       const int buf = kk & 1;
-      // prefetch: output-point data of plane z+AD, own column z+6, halo of plane z+HD
+      // prefetch: output-point data of plane z+AD, own column z+OA, halo of plane z+HD
       load_aux((kk + AD) % AR, z + AD);
-      ov[(kk + 10) % RQ] = val(z + 6 < ze + 4 ? own_idx(z + 6) : -1);
+      ov[(kk + 4 + OA) % RQ] = val(z + OA < ze + 4 ? own_idx(z + OA) : -1);
 #pragma unroll
       for (int i = 0; i < 2; ++i) hv[(kk + HD) % HR][i] = val(z + HD < ze ? halo_idx(z + HD, i) : -1);
       // stage plane z: tile centre and halo strips

@@ -51,47 +51,53 @@ int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool n
   return OCTHIP_SUCCESS;
 }
 template <int LPT, int LC>
-const void* kernel_of(int epi, bool nl) {
+const void* kernel_of(int epi, bool nl, bool multi) {
   if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
+      if (multi)
+        return nl ? reinterpret_cast<const void*>(&k_zmarch_multi<LPT, LC, EPI_TAYLOR, true>)
+                  : reinterpret_cast<const void*>(&k_zmarch_multi<LPT, LC, EPI_TAYLOR, false>);
       return nl ? reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_TAYLOR, true>)
                 : reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_TAYLOR, false>);
     } else {
       return nullptr;
     }
   }
+  if (multi)
+    return nl ? reinterpret_cast<const void*>(&k_zmarch_multi<LPT, LC, EPI_STORE, true>)
+              : reinterpret_cast<const void*>(&k_zmarch_multi<LPT, LC, EPI_STORE, false>);
   return nl ? reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_STORE, true>)
             : reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_STORE, false>);
 }
 }  // namespace
 
-// Workgroups of one k_zmarch instance that its registers alone let run on a CU (LDS not
-// counted): the segment-length choice keeps the LDS of that many resident.  Cached per
-// instance; 1 for shapes without an instance.
-int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl) {
-  static int cache[2][6][2][2] = {};  // [lpt-1][log2 lc][epi][nl], 0 = not asked yet
+// Workgroups of one z-march instance (k_zmarch, or k_zmarch_multi when multi) that its
+// registers alone let run on a CU (LDS not counted): the segment-length choice keeps the
+// LDS of that many resident.  Cached per instance; 1 for shapes without an instance.
+int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi) {
+  static int cache[2][2][6][2][2] = {};  // [multi][lpt-1][log2 lc][epi][nl], 0 = not asked yet
   int lg = 0;
   while ((1 << lg) < lc) ++lg;
   if (lpt < 1 || lpt > 2 || lg > 5 || (1 << lg) != lc || epi < 0 || epi > 1) return 1;
-  int& c = cache[lpt - 1][lg][epi][nl ? 1 : 0];
+  int& c = cache[multi ? 1 : 0][lpt - 1][lg][epi][nl ? 1 : 0];
   if (c > 0) return c;
   const void* f = nullptr;
   if (lpt == 2) {
     switch (lc) {
-      case 2: f = kernel_of<2, 2>(epi, nl); break;
-      case 4: f = kernel_of<2, 4>(epi, nl); break;
-      case 8: f = kernel_of<2, 8>(epi, nl); break;
-      case 16: f = kernel_of<2, 16>(epi, nl); break;
-      case 32: f = kernel_of<2, 32>(epi, nl); break;
+      case 2: f = kernel_of<2, 2>(epi, nl, multi); break;
+      case 4: f = kernel_of<2, 4>(epi, nl, multi); break;
+      case 8: f = kernel_of<2, 8>(epi, nl, multi); break;
+      case 16: f = kernel_of<2, 16>(epi, nl, multi); break;
+      case 32: f = multi ? nullptr : kernel_of<2, 32>(epi, nl, false); break;
       default: break;
     }
-  } else {
+  } else if (!multi) {
     switch (lc) {
-      case 1: f = kernel_of<1, 1>(epi, nl); break;
-      case 2: f = kernel_of<1, 2>(epi, nl); break;
-      case 4: f = kernel_of<1, 4>(epi, nl); break;
-      case 8: f = kernel_of<1, 8>(epi, nl); break;
-      case 16: f = kernel_of<1, 16>(epi, nl); break;
+      case 1: f = kernel_of<1, 1>(epi, nl, false); break;
+      case 2: f = kernel_of<1, 2>(epi, nl, false); break;
+      case 4: f = kernel_of<1, 4>(epi, nl, false); break;
+      case 8: f = kernel_of<1, 8>(epi, nl, false); break;
+      case 16: f = kernel_of<1, 16>(epi, nl, false); break;
       default: break;
     }
   }
