@@ -228,7 +228,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   b->order = order;
   b->dt = deltat;
   b->nl = all_nl;
-  b->lc = std::min(16, minL);
+  b->lc = std::min(std::min(16, 2048 / ZNC), minL);  // lanes per workgroup: <= 1024 threads
   b->lpt = 2;
   b->mfma = minL >= 16;
   for (int i = 0; i < npairs; ++i) b->mfma = b->mfma && hms[i]->use_mfma;

@@ -124,10 +124,10 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
       lo[d] = std::min(lo[d], c[3 * (size_t)q + d]);
       hi[d] = std::max(hi[d], c[3 * (size_t)q + d]);
     }
-  op->ntx = (hi[0] - lo[0] + ZT) / ZT;
-  op->nty = (hi[1] - lo[1] + ZT) / ZT;
-  op->mx = op->ntx * ZT + 2 * ZR;
-  op->my = op->nty * ZT + 2 * ZR;
+  op->ntx = (hi[0] - lo[0] + ZTX) / ZTX;
+  op->nty = (hi[1] - lo[1] + ZTY) / ZTY;
+  op->mx = op->ntx * ZTX + 2 * ZR;
+  op->my = op->nty * ZTY + 2 * ZR;
   op->mz = (hi[2] - lo[2] + 1) + 2 * ZPADZ;
   const int ox = lo[0] - ZR, oy = lo[1] - ZR, oz = lo[2] - ZPADZ;
   const size_t nsite = (size_t)op->mx * op->my * op->mz;
@@ -142,7 +142,7 @@ int build_march(octhip_nl_operator_s* op, hipStream_t s) {
     if (x < 0 || y < 0 || z < 0 || x >= op->mx || y >= op->my || z >= op->mz) continue;
     mc[(size_t)x + (size_t)op->mx * ((size_t)y + (size_t)op->my * z)] = q;
     if (q < np) {
-      const int t = (int)((x - ZR) / ZT + op->ntx * ((y - ZR) / ZT));
+      const int t = (int)((x - ZR) / ZTX + op->ntx * ((y - ZR) / ZTY));
       op->tile_zlo[t] = std::min(op->tile_zlo[t], (int)z);
       op->tile_zhi[t] = std::max(op->tile_zhi[t], (int)z);
     }
@@ -208,10 +208,11 @@ int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident
 // lane chunk and lanes per thread of a z-march launch over rows of L doubles
 void zmarch_shape(const octhip_nl_operator_s* op, int L, int* lc, int* lpt) {
   int c = op->lane_chunk > 0 ? op->lane_chunk : 16;
-  if (c > L) c = L;
   int t = op->lanes_per_thread > 0 ? op->lanes_per_thread : 2;
+  while (c > 1 && ZNC * c / 2 > 1024) c /= 2;  // 1024 threads per workgroup at most
+  if (c > L) c = L;
   if (t > c) t = c;
-  if (c == 32) t = 2;  // 1024 threads per workgroup at most
+  if (ZNC * c / t > 1024) t = 2;
   *lc = c;
   *lpt = t;
 }
@@ -267,28 +268,28 @@ int zmarch_items(octhip_nl_operator_s* op, int zs) {
         items.push_back(make_int4(tx, ty, zb, ze));
       }
   // Index table of every item, read by the kernel's prologue as one coalesced block: for
-  // planes P = zb-4 .. ze+3, 192 sites each (the 64 tile columns, then the 128 halo sites
+  // planes P = zb-4 .. ze+3, ZS sites each (the ZNC tile columns, then the ZNH halo sites
   // of the cross-shaped window in the kernel's slot order), the point index at the site
   // or -1; halo entries only for the item's own planes zb .. ze-1.
   std::vector<int> tab, toff(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
     const int4 it = items[i];
     toff[i] = (int)tab.size();
-    const long x0 = (long)it.x * ZT + ZR, y0 = (long)it.y * ZT + ZR;
+    const long x0 = (long)it.x * ZTX + ZR, y0 = (long)it.y * ZTY + ZR;
     for (int P = it.z - 4; P < it.w + 4; ++P)
-      for (int t = 0; t < 192; ++t) {
+      for (int t = 0; t < ZS; ++t) {
         long hx, hy;
-        if (t < 64) {
-          hx = x0 + (t & 7);
-          hy = y0 + (t >> 3);
+        if (t < ZNC) {
+          hx = x0 + t % ZTX;
+          hy = y0 + t / ZTX;
         } else {
-          const int h = t - 64;
-          if (h < 32) { hx = x0 - 4 + (h & 3); hy = y0 + (h >> 2); }
-          else if (h < 64) { hx = x0 + 8 + ((h - 32) & 3); hy = y0 + ((h - 32) >> 2); }
-          else if (h < 96) { hx = x0 + ((h - 64) & 7); hy = y0 - 4 + ((h - 64) >> 3); }
-          else { hx = x0 + ((h - 96) & 7); hy = y0 + 8 + ((h - 96) >> 3); }
+          const int h = t - ZNC;
+          if (h < 4 * ZTY) { hx = x0 - 4 + h % 4; hy = y0 + h / 4; }
+          else if (h < 8 * ZTY) { hx = x0 + ZTX + (h - 4 * ZTY) % 4; hy = y0 + (h - 4 * ZTY) / 4; }
+          else if (h < 8 * ZTY + 4 * ZTX) { hx = x0 + (h - 8 * ZTY) % ZTX; hy = y0 - 4 + (h - 8 * ZTY) / ZTX; }
+          else { hx = x0 + (h - 8 * ZTY - 4 * ZTX) % ZTX; hy = y0 + ZTY + (h - 8 * ZTY - 4 * ZTX) / ZTX; }
         }
-        const bool need = t < 64 || (P >= it.z && P < it.w);
+        const bool need = t < ZNC || (P >= it.z && P < it.w);
         tab.push_back(need ? op->mcube[(size_t)hx + (size_t)op->mx * ((size_t)hy + (size_t)op->my * P)] : -1);
       }
   }

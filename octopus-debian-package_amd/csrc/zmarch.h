@@ -25,10 +25,20 @@
 
 namespace octhip {
 
-constexpr int ZT = 8;                // tile columns per side
-constexpr int ZR = 4;                // stencil reach
-constexpr int ZW = ZT + 2 * ZR;      // window side (tile + halo)
-constexpr int ZS = 192;              // sites of the cross-shaped plane window: 64 tile + 128 halo
+#ifndef ZM_TX
+#define ZM_TX 8
+#endif
+#ifndef ZM_TY
+#define ZM_TY 8
+#endif
+constexpr int ZTX = ZM_TX;            // tile columns along x
+constexpr int ZTY = ZM_TY;            // tile columns along y
+constexpr int ZR = 4;                 // stencil reach
+constexpr int ZNC = ZTX * ZTY;        // own columns of a tile
+constexpr int ZNH = 8 * (ZTX + ZTY);  // halo sites of the cross-shaped plane window
+constexpr int ZS = ZNC + ZNH;         // sites of the plane window
+constexpr int ZHU = (ZNH + ZNC - 1) / ZNC;  // halo sites loaded per column thread group
+static_assert((ZTX & (ZTX - 1)) == 0 && (ZTY & (ZTY - 1)) == 0 && ZS % 4 == 0, "tile shape");
 constexpr int ZPADZ = 8;             // march-cube padding below/above the inner points in z
 // Prefetch depths (planes loaded ahead) of the own column values, the halo values and the
 // output-point data, and the own-value ring length (= z loop unroll, >= own depth + 5).
@@ -101,7 +111,7 @@ struct ZArgs {
 template <int LPT, int LC, int EPI, bool NL>
 __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, const int item, const int chunk) {
   constexpr int PAIRS = LC / LPT;    // threads per column
-  constexpr int NT = 64 * PAIRS;
+  constexpr int NT = ZNC * PAIRS;
   extern __shared__ double2 zsh2[];  // 16-byte aligned: row chunks move as ds_*_b128
   double* const zsh = reinterpret_cast<double*>(zsh2);
   double* const lds0 = zsh;                              // plane buffer 0
@@ -119,7 +129,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   for (int j = 0; j < 25; ++j) a_w[j] = A.w[j];
   const int kp = threadIdx.x % PAIRS;
   const int col = threadIdx.x / PAIRS;
-  const int cx = col & 7, cy = col >> 3;
+  const int cx = col % ZTX, cy = col / ZTX;
   const int l = chunk * LC + kp * LPT;         // first lane of this thread
   const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
   // row q of an array starts at byte q << sh; arrays without a role here get no records
@@ -134,25 +144,29 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const auto r_y = zrsrc(A.yout, A.yout ? np_bytes : 0u);
   const auto r_acc = zrsrc(a_acc, has_acc ? np_bytes : 0u);
   auto roff = [&](int q) { return ((uint32_t)q << sh) + lbyte; };
-  // two halo units per thread: site slot 64 + col + 64*i, same lane pair as the thread
-  int hlds[2];
+  // halo units of this thread: halo site h = col + ZNC*i (if h < ZNH), same lane pair
+  int hlds[ZHU];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) hlds[i] = (64 + col + 64 * i) * LC + kp * LPT;
-  // Cross-shaped window in LDS, one LC-lane chunk per site slot ((x0, y0) = first tile column):
-  //   slot c (0..63)          tile column c = cx + 8 cy
-  //   slot 64 + 4 r + k       left strip,  x = x0-4+k, y = y0+r   (k < 4, r < 8)
-  //   slot 96 + 4 r + k       right strip, x = x0+8+k, y = y0+r
-  //   slot 128 + 8 k + c      bottom strip, x = x0+c, y = y0-4+k
-  //   slot 160 + 8 k + c      top strip,    x = x0+c, y = y0+8+k
+  for (int i = 0; i < ZHU; ++i) hlds[i] = (ZNC + col + ZNC * i) * LC + kp * LPT;
+  auto halo_ok = [&](int i) { return ZNC * (i + 1) <= ZNH || col + ZNC * i < ZNH; };
+  // Cross-shaped window in LDS, one LC-lane chunk per site slot ((x0, y0) = first tile
+  // column, W = ZTX, H = ZTY, N = W H):
+  //   slot c (0..N-1)               tile column c = cx + W cy
+  //   slot N + 4 r + k              left strip,   x = x0-4+k, y = y0+r   (k < 4, r < H)
+  //   slot N + 4 H + 4 r + k        right strip,  x = x0+W+k, y = y0+r
+  //   slot N + 8 H + W k + c        bottom strip, x = x0+c,   y = y0-4+k (c < W)
+  //   slot N + 8 H + 4 W + W k + c  top strip,    x = x0+c,   y = y0+H+k
   const int own_lds = col * LC + kp * LPT;
   auto xnb = [&](int dx) {  // LDS offset of the x-neighbour dx of this thread's column
     const int xx = cx + dx;
-    const int slot = xx < 0 ? 64 + 4 * cy + (xx + 4) : (xx >= 8 ? 96 + 4 * cy + (xx - 8) : xx + 8 * cy);
+    const int slot = xx < 0 ? ZNC + 4 * cy + (xx + 4)
+                            : (xx >= ZTX ? ZNC + 4 * ZTY + 4 * cy + (xx - ZTX) : xx + ZTX * cy);
     return slot * LC + kp * LPT;
   };
   auto ynb = [&](int dy) {  // LDS offset of the y-neighbour dy
     const int yy = cy + dy;
-    const int slot = yy < 0 ? 128 + 8 * (yy + 4) + cx : (yy >= 8 ? 160 + 8 * (yy - 8) + cx : cx + 8 * yy);
+    const int slot = yy < 0 ? ZNC + 8 * ZTY + ZTX * (yy + 4) + cx
+                            : (yy >= ZTY ? ZNC + 8 * ZTY + 4 * ZTX + ZTX * (yy - ZTY) + cx : cx + ZTX * yy);
     return slot * LC + kp * LPT;
   };
 
@@ -168,16 +182,16 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   };
 
   const int zb = it.z, ze = it.w;
-  int* const tks = tidx + (ze - zb + 8) * 192;  // ket slots of the own sites
-  // ---- index table of the item in LDS: 192 sites per plane (64 own columns, then the
-  //      128 halo sites), planes zb-4 .. ze+3; halo entries exist for planes zb .. ze-1.
+  int* const tks = tidx + (ze - zb + 8) * ZS;  // ket slots of the own sites
+  // ---- index table of the item in LDS: ZS sites per plane (the ZNC own columns, then the
+  //      ZNH halo sites), planes zb-4 .. ze+3; halo entries exist for planes zb .. ze-1.
   //      Precomputed per item, so the prologue is one coalesced block copy; index reads
   //      then cost LDS, not the in-order global-load counter.
   {
     // (through a buffer resource: the descriptors of a batched launch live in global
     // memory, and a generic pointer loaded from there would be read with flat loads that
     // the compiler must order against the LDS stores)
-    const int n4 = (ze - zb + 8) * 192 / 4;
+    const int n4 = (ze - zb + 8) * ZS / 4;
     const auto r_tab = zrsrc(za.tab + za.tab_off[item], (uint32_t)n4 * 16u);
     int4* const t4 = reinterpret_cast<int4*>(tidx);
     for (int base = 0; base < n4; base += NT * 8) {
@@ -195,8 +209,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
     }
   }
   __syncthreads();
-  auto own_idx = [&](int P) { return tidx[(P - zb + 4) * 192 + col]; };
-  auto halo_idx = [&](int P, int i) { return tidx[(P - zb + 4) * 192 + 64 + col + 64 * i]; };
+  auto own_idx = [&](int P) { return tidx[(P - zb + 4) * ZS + col]; };
+  auto halo_idx = [&](int P, int i) { return halo_ok(i) ? tidx[(P - zb + 4) * ZS + ZNC + col + ZNC * i] : -1; };
 
   // Rings with static slots (the z loop is unrolled by RQ, so every slot index is a
   // compile-time constant and values stay in the registers their loads wrote):
@@ -211,7 +225,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   constexpr int AR = ring_above(RQ, AD);
   static_assert(OA >= 5 && RQ >= OA + 5 && HR > HD && AR > AD, "ring sizes");
   Vec<LPT> ov[RQ];
-  Vec<LPT> hv[HR][2];
+  Vec<LPT> hv[HR][ZHU];
   double vr[AR] = {};
   Vec<LPT> kr[AR];
   Vec<LPT> ar[AR];
@@ -225,7 +239,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
     const int qo = (P < ze && (unsigned)q < (unsigned)a_np) ? q : -1;
     vr[slot] = bld<1>(r_v, (uint32_t)qo << 3).v[0];
     if constexpr (NL) {
-      const int ks = tks[(P - zb + 4) * 64 + col];
+      const int ks = tks[(P - zb + 4) * ZNC + col];
       kr[slot] = bld<LPT>(r_k, roff(qo >= 0 ? ks : -1));
     }
     if constexpr (EPI == EPI_TAYLOR) ar[slot] = bld<LPT>(r_ai, roff(qo));
@@ -235,18 +249,18 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
 #pragma unroll
   for (int k = 0; k < HD; ++k)
 #pragma unroll
-    for (int i = 0; i < 2; ++i) hv[k][i] = val(zb + k < ze ? halo_idx(zb + k, i) : -1);
+    for (int i = 0; i < ZHU; ++i) hv[k][i] = val(zb + k < ze ? halo_idx(zb + k, i) : -1);
   // ket slots of the own sites, looked up while the first planes' values are in flight
   // (one unconditional range-checked load per entry; -1 where the site has no point)
   if constexpr (NL) {
     const auto r_ks = zrsrc(a_kslot, (uint32_t)a_np * 4u);
-    const int totk = (ze - zb + 8) * 64;
+    const int totk = (ze - zb + 8) * ZNC;
     for (int base = 0; base < totk; base += NT * 8) {
       int kq[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = min(base + (int)threadIdx.x + u * NT, totk - 1);
-        const int q = tidx[(e >> 6) * 192 + (e & 63)];
+        const int q = tidx[(e / ZNC) * ZS + (e % ZNC)];
         const int k = (int)__builtin_amdgcn_raw_buffer_load_b32(r_ks, (uint32_t)q << 2, 0, 0);
         kq[u] = (unsigned)q < (unsigned)a_np ? k : -1;
       }
@@ -271,11 +285,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       load_aux((kk + AD) % AR, z + AD);
       ov[(kk + 4 + OA) % RQ] = val(z + OA < ze + 4 ? own_idx(z + OA) : -1);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) hv[(kk + HD) % HR][i] = val(z + HD < ze ? halo_idx(z + HD, i) : -1);
+      for (int i = 0; i < ZHU; ++i) hv[(kk + HD) % HR][i] = val(z + HD < ze ? halo_idx(z + HD, i) : -1);
       // stage plane z: tile centre and halo strips
       lds_st(buf, own_lds, ov[(kk + 4) % RQ]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) lds_st(buf, hlds[i], hv[kk % HR][i]);
+      for (int i = 0; i < ZHU; ++i)
+        if (halo_ok(i)) lds_st(buf, hlds[i], hv[kk % HR][i]);
       __syncthreads();
       // output point (x0+cx, y0+cy, z)
       const int q0 = own_idx(z);
@@ -337,7 +352,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
 }
 
 template <int LPT, int LC, int EPI, bool NL>
-__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
+__global__ __launch_bounds__(ZNC * LC / LPT) void k_zmarch(const ZArgs za) {
   const int2 sc = za.sched[blockIdx.x];
   if (sc.x < 0) return;              // padding entry of the schedule (whole workgroup)
   zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.x], sc.x, sc.y);
@@ -346,7 +361,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch(const ZArgs za) {
 // Several operators (systems) and batches in one launch: sched[b] = {descriptor, item,
 // lane chunk, -}; descriptors zs[] are ZArgs in device memory (their sched is unused).
 template <int LPT, int LC, int EPI, bool NL>
-__global__ __launch_bounds__(64 * LC / LPT) void k_zmarch_multi(const ZArgs* __restrict__ zs,
+__global__ __launch_bounds__(ZNC * LC / LPT) void k_zmarch_multi(const ZArgs* __restrict__ zs,
                                                                 const int4* __restrict__ sched) {
   const int4 sc = sched[blockIdx.x];
   if (sc.x < 0) return;
@@ -356,7 +371,7 @@ __global__ __launch_bounds__(64 * LC / LPT) void k_zmarch_multi(const ZArgs* __r
 
 // dynamic LDS bytes of k_zmarch for a lane chunk and segment length
 inline size_t zmarch_shmem(int lc, int zseg) {
-  return sizeof(double) * 2 * ZS * lc + sizeof(int) * (size_t)(zseg + 8) * (192 + 64);
+  return sizeof(double) * 2 * ZS * lc + sizeof(int) * (size_t)(zseg + 8) * (ZS + ZNC);
 }
 
 }  // namespace octhip

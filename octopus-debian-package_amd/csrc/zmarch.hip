@@ -6,8 +6,10 @@ namespace octhip {
 namespace {
 template <int LPT, int LC, bool NL>
 int launch_nl(const ZArgs& z, int epi, int grid, size_t shmem, hipStream_t s) {
-  constexpr int NT = 64 * LC / LPT;
-  if (epi == EPI_TAYLOR) {
+  constexpr int NT = ZNC * LC / LPT;
+  if constexpr (NT > 1024) {
+    return fail(OCTHIP_ERR_INVALID, "z-march: %d lanes per workgroup exceed 1024 threads", LC);
+  } else if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
       hipLaunchKernelGGL((k_zmarch<LPT, LC, EPI_TAYLOR, NL>), dim3(grid), dim3(NT), shmem, s, z);
     } else {
@@ -32,10 +34,12 @@ int launch_lc(const ZArgs& z, int epi, int grid, int zseg, hipStream_t s) {
 }
 template <int LPT, int LC>
 int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool nl, int zseg, hipStream_t s) {
-  constexpr int NT = 64 * LC / LPT;
+  constexpr int NT = ZNC * LC / LPT;
   const size_t shmem = zmarch_shmem(LC, zseg);
   ProfScope prof("k_apply", s);
-  if (epi == EPI_TAYLOR) {
+  if constexpr (NT > 1024) {
+    return fail(OCTHIP_ERR_INVALID, "z-march: %d lanes per workgroup exceed 1024 threads", LC);
+  } else if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
       if (nl) hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_TAYLOR, true>), dim3(len), dim3(NT), shmem, s, zs, sched);
       else hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_TAYLOR, false>), dim3(len), dim3(NT), shmem, s, zs, sched);
@@ -52,7 +56,9 @@ int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool n
 }
 template <int LPT, int LC>
 const void* kernel_of(int epi, bool nl, bool multi) {
-  if (epi == EPI_TAYLOR) {
+  if constexpr (ZNC * LC / LPT > 1024) {
+    return nullptr;
+  } else if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
       if (multi)
         return nl ? reinterpret_cast<const void*>(&k_zmarch_multi<LPT, LC, EPI_TAYLOR, true>)
@@ -102,7 +108,7 @@ int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi) {
     }
   }
   int nb = 1;
-  if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64 * lc / lpt, 0) != hipSuccess || nb < 1) nb = 1;
+  if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, ZNC * lc / lpt, 0) != hipSuccess || nb < 1) nb = 1;
   c = nb;
   return nb;
 }
