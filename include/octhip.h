@@ -130,6 +130,12 @@ int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double mass, const d
                               const int* maps, const double* scals,
                               octhip_hamiltonian_t* hm);
 int octhip_hamiltonian_destroy(octhip_hamiltonian_t hm);
+/* Release the projector and Taylor work buffers this Hamiltonian keeps for `stream` (one
+ * set per stream that calls it, so batches on different streams run concurrently).  No
+ * counterpart in the reference: for callers that retire streams, whose sets would
+ * otherwise stay allocated until destroy.  Synchronises the stream first; buffers
+ * captured into a Taylor graph are not affected. */
+int octhip_hamiltonian_release_stream(octhip_hamiltonian_t hm, void* stream);
 /* replace the local potential (np doubles, host) -- hamiltonian_elec_update */
 int octhip_hamiltonian_set_potential(octhip_hamiltonian_t hm, const double* vlocal, void* stream);
 /* v_eff(p) = vlocal(p) + sum_d field[d] * x_d(p): the length-gauge dipole term of

@@ -365,6 +365,24 @@ extern "C" int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t h, const
   return update_veff(h, s);
 }
 
+extern "C" int octhip_hamiltonian_release_stream(octhip_hamiltonian_t h, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  const hipStream_t s = (hipStream_t)stream;
+  for (size_t i = 0; i < h->sets.size(); ++i) {
+    if (h->sets[i].s != s) continue;
+    OCTHIP_HIP(hipStreamSynchronize(s));
+    const auto w = h->sets[i];
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1}) dfree(p);
+    h->sets.erase(h->sets.begin() + (long)i);
+    // the mirrored pointers may name the freed set: they are rebound by the next call
+    h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = nullptr;
+    h->work_ldf = h->y_ldf = -1;
+    break;
+  }
+  return OCTHIP_SUCCESS;
+}
+
 extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
                                               int is_complex, int terms, int set_bc, void* stream) {
   OCTHIP_REQUIRE_INIT();

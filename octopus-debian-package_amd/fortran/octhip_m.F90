@@ -25,6 +25,7 @@ module octhip_m
     octhip_malloc, octhip_free, octhip_memcpy_htod, octhip_memcpy_dtoh, octhip_stream_synchronize, &
     octhip_nl_operator_create, octhip_nl_operator_destroy, octhip_nl_operator_operate_batch, &
     octhip_operate_ri_vec, octhip_hamiltonian_create, octhip_hamiltonian_destroy, &
+    octhip_hamiltonian_release_stream, &
     octhip_hamiltonian_set_potential, octhip_hamiltonian_set_dipole_field, &
     octhip_hamiltonian_apply_batch, octhip_exponential_taylor_batch, &
     octhip_taylor_graph_create, octhip_taylor_graph_launch, octhip_taylor_graph_destroy, &
@@ -118,6 +119,11 @@ module octhip_m
     integer(c_int) function octhip_hamiltonian_destroy(hm) bind(C, name="octhip_hamiltonian_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: hm
+    end function
+    integer(c_int) function octhip_hamiltonian_release_stream(hm, stream) &
+        bind(C, name="octhip_hamiltonian_release_stream")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, stream
     end function
     integer(c_int) function octhip_hamiltonian_set_potential(hm, vlocal, stream) &
         bind(C, name="octhip_hamiltonian_set_potential")

@@ -43,6 +43,7 @@ SIGNATURES = {
     "octhip_operate_ri_vec": [_I, _DP, _I, _P, _P, _P, _P, _I, _P, _P],
     "octhip_hamiltonian_create": [_P, _D, _DP, _I, _IP, _DP, _IP, _DP, C.POINTER(_P)],
     "octhip_hamiltonian_destroy": [_P],
+    "octhip_hamiltonian_release_stream": [_P, _P],
     "octhip_hamiltonian_set_potential": [_P, _DP, _P],
     "octhip_hamiltonian_set_dipole_field": [_P, _DP, _DP, _P],
     "octhip_hamiltonian_apply_batch": [_P, _P, _P, _I, _I, _I, _I, _P],

@@ -182,6 +182,10 @@ class HamiltonianDevice:
         call("octhip_hamiltonian_set_dipole_field", self.handle, as_ptr(x, C.c_double),
              as_ptr(f, C.c_double), _stream())
 
+    def release_stream(self) -> None:
+        """Free the work buffers this Hamiltonian keeps for the current stream."""
+        call("octhip_hamiltonian_release_stream", self.handle, _stream())
+
     def __del__(self):
         if getattr(self, "handle", None):
             try:

@@ -4,7 +4,9 @@ batch, td/propagation_ops_elec.F90:236, accel_set_stream(ib)).
 Every stream gets its own projector and Taylor work buffers, so H-apply and Taylor steps
 issued on two streams at once must be bit-identical to the same calls issued one after
 another on one stream.  A Taylor graph keeps the buffers it was captured with: eager
-steps on the capturing stream between replays must not disturb the replays.
+steps on the capturing stream between replays must not disturb the replays.  Releasing a
+stream's buffers (octhip_hamiltonian_release_stream) lets the next call on it allocate a
+fresh set with the same results.
 """
 import pytest
 
@@ -79,3 +81,30 @@ def test_graph_keeps_its_buffers(gpu):
             te.apply_batch(hm, a, 0.01)
             torch.cuda.synchronize()
             assert torch.equal(twin.ff[: twin.nrows], a.ff[: a.nrows]), f"replay {rep}"
+
+
+def test_release_stream_frees_and_reallocates(gpu):
+    print("[release stream]", flush=True)
+    torch = gpu
+    s = system()
+    hm = api.HamiltonianDevice.from_system(s)
+    b, = batches(s, 1)
+    ref = api.Batch(b.nst, s.np, True, b.pack_size)
+    api.hamiltonian_elec_apply_batch(hm, b, ref)
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    te = api.Exponential(4)
+    x = b.clone()
+    with torch.cuda.stream(st):
+        te.apply_batch(hm, x, 0.01)        # allocates the stream's Taylor buffers
+        hm.release_stream()                # frees them (after synchronising the stream)
+        y = b.clone()
+        te.apply_batch(hm, y, 0.01)        # allocates a new set
+        out = api.Batch(b.nst, s.np, True, b.pack_size)
+        api.hamiltonian_elec_apply_batch(hm, b, out)
+        hm.release_stream()
+        hm.release_stream()                # nothing left to release: no error
+    torch.cuda.synchronize()
+    assert torch.equal(x.ff[: x.nrows], y.ff[: y.nrows])
+    assert torch.equal(out.ff, ref.ff)
+
