@@ -153,6 +153,21 @@ def test_hamiltonian_workload_batch(gpu):
     assert np.array_equal(got_z, got_g)
 
 
+def test_taylor_step_workload_batch(gpu):
+    """One Taylor-4 step (dt 0.02) of a 32-state batch of the R=12, h=0.25 sphere workload
+    with the production launch shapes (z-march Taylor epilogue, automatic segments)."""
+    print("[taylor sphere R=12 batch]", flush=True)
+    s = workloads.hamiltonian_system()
+    z = workloads.gaussian_envelope_states(s, 32)
+    psi = workloads.pack(z, s.np_part)
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(psi, 0.02)
+    hm = api.HamiltonianDevice.from_system(s)
+    psib = api.Batch.from_states(z, s.np_part)
+    api.Exponential(4).apply_batch(hm, psib, 0.02)
+    gpu.cuda.synchronize()
+    assert max_rel_err(psib.ff.cpu().numpy()[: s.np], psi[: s.np]) <= 1e-12
+
+
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("imag_time", [False, True])
 @pytest.mark.parametrize("kernel,lpt", [(1, 0), (2, 2), (2, 1)])
