@@ -113,7 +113,7 @@ bool is_star4(int n, const int* pts) {
   return true;
 }
 
-// March cube: the inner points' bounding box rounded up to whole 8x8 column tiles in x
+// March cube: the inner points' bounding box rounded up to whole ZTX x ZTY column tiles in x
 // and y, padded by the stencil reach in x/y and by ZPADZ planes in z; site -> point index.
 int build_march(octhip_nl_operator_s* op, hipStream_t s) {
   const int np = op->np, np_part = op->np_part;

@@ -1,21 +1,22 @@
 // 2.5-D "z-march" apply for the order-4 star stencil on a lattice (gfx950).
 //
 // The gather kernel (k_apply) reads 25 full rows per point, so L2 has to deliver 25x the
-// HBM traffic.  Here a workgroup owns an 8x8 tile of lattice columns and one chunk of LC
-// lanes, and walks the tile's z range plane by plane:
+// HBM traffic.  Here a workgroup owns a ZM_TX x ZM_TY tile of lattice columns (8x8 by
+// default) and one chunk of LC lanes, and walks a segment of the tile's z range plane by
+// plane:
 //   * the current plane (tile + 4-site halo strips on the four sides) is staged in LDS
 //     (double-buffered): x and y neighbours are LDS reads;
 //   * each thread keeps its column's values at z-4..z+4 in registers: z neighbours cost
 //     nothing;
-//   * the site -> point indices of the item (and the ket slots of its own sites) are
-//     staged in LDS once per item;
+//   * the site -> point indices of the item (precomputed per item on the host) and the
+//     ket slots of its own sites are staged in LDS once per item;
 //   * loads run a few planes ahead (own column values, halo values, and the potential,
 //     ket row and Taylor accumulator of the output point; depths in ZRings), into ring buffers
 //     with compile-time slots (the z loop is unrolled by the ring period); every load is
 //     unconditional, so the compiler's wait counts stay exact and a step waits only on
 //     loads issued planes earlier.
-// Per output point the tile reads 1 own row chunk + 2 halo chunks (128 halo sites for 64
-// columns) instead of 25 rows.  The lattice is a padded "march cube" of point indices;
+// Per output point an 8x8 tile reads 1 own row chunk + 2 halo chunks (128 halo sites for
+// 64 columns) instead of 25 rows; rows go through raw buffer resources (below).  The lattice is a padded "march cube" of point indices;
 // sites that are not points hold -1.  Values follow the gather kernel exactly: point
 // q < lim -> its row, anything else -> 0, and the 25 terms are summed in stencil order
 // (x -4..4, y -4..-1,1..4, z -4..-1,1..4) with fma, so results are bit-identical to it.
@@ -100,7 +101,7 @@ constexpr int ring_above(int rq, int d) {
 struct ZArgs {
   ApplyArgs a;
   const int4* __restrict__ items;    // {tile x, tile y, first plane, end plane} (cube z)
-  const int* __restrict__ tab;       // per item: index table of planes zb-4 .. ze+3, 192 sites each
+  const int* __restrict__ tab;       // per item: index table of planes zb-4 .. ze+3, ZS sites each
   const int* __restrict__ tab_off;   // per item: first int of its table (a multiple of 4)
   const int2* __restrict__ sched;    // blockIdx -> {item, lane chunk}
 };
