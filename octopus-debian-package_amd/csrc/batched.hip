@@ -42,7 +42,7 @@ struct Pair {
   double* partial = nullptr;
   double* proj = nullptr;
   double* ksum = nullptr;
-  double* y[2] = {nullptr, nullptr};
+  double* y[3] = {nullptr, nullptr, nullptr};  // Taylor powers (three for order 4)
   const int4* items = nullptr;   // the operator's z-march items for this pair's segment length
   const int* tab = nullptr;      // and their index tables
   const int* tab_off = nullptr;
@@ -100,6 +100,7 @@ void destroy(octhip_batched_s* b) {
     dfree(p.ksum);
     dfree(p.y[0]);
     dfree(p.y[1]);
+    dfree(p.y[2]);
   }
   free_seg(b->zero);
   free_seg(b->reduce);
@@ -143,6 +144,13 @@ int build_bra(octhip_batched_s* b, const std::vector<const double*>& in, Seg& ou
   return make_seg(args, blocks, out);
 }
 
+// input of pass k (0 = the H-apply): y_{k-1} (y_0 = psi); orders 2..4 keep y_j in y[j-1],
+// higher orders ping-pong two buffers
+const double* pass_input(const Pair& p, int k, int order) {
+  if (k <= 1) return p.psi;
+  return taylor_deferred(order) ? p.y[k - 2] : p.y[(k - 2) & 1];
+}
+
 // z-march descriptors of every pair for one pass
 int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
   std::vector<ZArgs> zs(b->pairs.size());
@@ -151,10 +159,31 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
     const octhip_nl_operator_s* op = p.h->kin;
     ZArgs& z = zs[i];
     std::memset(&z, 0, sizeof(z));
-    const double* in = (k <= 1) ? p.psi : p.y[(k - 2) & 1];
+    const double* in = pass_input(p, k, b->order);
     h_apply_args(p.h, in, p.L, p.ksum, z.a);
+    const bool deferred = taylor_deferred(b->order);
     if (k == 0) {
       z.a.fo = p.hpsi;
+    } else if (deferred && k < b->order) {
+      z.a.fo = p.y[k - 1];  // y_k, stored (no accumulator in between)
+    } else if (deferred) {
+      // last order: psi + z_1 y_1 + ... + z_order H y_{order-1}, terms added in order k
+      z.a.acc = p.psi;
+      if (k >= 3) {
+        z.a.ya = p.y[0];
+        z.a.car = b->zf[1].real();
+        z.a.cai = b->zf[1].imag();
+      }
+      if (k >= 4) {
+        z.a.yb = p.y[1];
+        z.a.cbr = b->zf[2].real();
+        z.a.cbi = b->zf[2].imag();
+      }
+      z.a.has_c1 = 1;
+      z.a.c1r = b->zf[k - 1].real();
+      z.a.c1i = b->zf[k - 1].imag();
+      z.a.c2r = b->zf[k].real();
+      z.a.c2i = b->zf[k].imag();
     } else {
       z.a.yout = (k < b->order) ? p.y[(k - 1) & 1] : nullptr;
       if (k == 1) {
@@ -174,8 +203,9 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
     z.sched = nullptr;
     // every pointer the kernel dereferences must exist
     // (the H-apply descriptor may lack fo: octhip_batched_apply refuses to run without hpsi)
-    const bool ok = z.a.fi && z.items && z.tab && z.tab_off &&
-                    (!z.a.kslot || z.a.ksum) && (k < 2 || z.a.acc) && (k == 0 || k == b->order || z.a.yout);
+    const bool ok = z.a.fi && z.items && z.tab && z.tab_off && (!z.a.kslot || z.a.ksum) &&
+                    (deferred ? (k == 0 || (k < b->order ? z.a.fo != nullptr : z.a.acc != nullptr))
+                              : ((k < 2 || z.a.acc) && (k == 0 || k == b->order || z.a.yout)));
     if (!ok) return fail(OCTHIP_ERR_STATE, "batched: incomplete descriptor for pair %zu, pass %d", i, k);
     if (!zmarch_fits(op, z.a))
       return fail(OCTHIP_ERR_INVALID, "batched: pair %zu is too large for 32-bit row offsets (%d rows of %d doubles)",
@@ -252,6 +282,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     }
     if (!rc) rc = dalloc((void**)&p.y[0], sizeof(double) * h->np * L);
     if (!rc) rc = dalloc((void**)&p.y[1], sizeof(double) * h->np * L);
+    if (!rc && taylor_deferred(order) && order == 4) rc = dalloc((void**)&p.y[2], sizeof(double) * h->np * L);
     // the items serve both epilogues: segments for the residency both instances allow
     if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident));
     if (!rc) {
@@ -316,7 +347,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   // bra tables and z-march descriptors: [0] H-apply, [k] Taylor order k
   for (int k = 0; k <= order && !rc; ++k) {
     std::vector<const double*> in;
-    for (const Pair& p : b->pairs) in.push_back(k <= 1 ? p.psi : p.y[(k - 2) & 1]);
+    for (const Pair& p : b->pairs) in.push_back(pass_input(p, k, order));
     b->bra.emplace_back();
     rc = build_bra(b, in, b->bra.back());
     ZArgs* zd = nullptr;
@@ -370,7 +401,9 @@ extern "C" int octhip_batched_taylor_step(octhip_batched_t b, void* stream) {
   OCTHIP_REQUIRE(b != nullptr, "null handle");
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_multi(MULTI_ZERO, b->zero.args, b->zero.off, b->zero.nseg, b->zero.nblocks, s);
-  for (int k = 1; k <= b->order && !rc; ++k) rc = run_pass(b, k, EPI_TAYLOR, s);
+  const bool deferred = taylor_deferred(b->order);
+  for (int k = 1; k <= b->order && !rc; ++k)
+    rc = run_pass(b, k, (deferred && k < b->order) ? EPI_STORE : EPI_TAYLOR, s);
   return rc;
 }
 

@@ -23,7 +23,7 @@ enum { O_POINTS = 0, O_PROJS, O_MATRIX, O_MAP, O_SCAL, O_MIX };
 octhip_hamiltonian_s::WorkSet& workset(octhip_hamiltonian_s* h, hipStream_t s) {
   for (auto& w : h->sets)
     if (w.s == s) return w;
-  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr});
+  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr});
   return h->sets.back();
 }
 
@@ -33,6 +33,7 @@ void bind(octhip_hamiltonian_s* h, const octhip_hamiltonian_s::WorkSet& w) {
   h->d_ksum = w.ksum;
   h->d_y[0] = w.y0;
   h->d_y[1] = w.y1;
+  h->d_y[2] = w.y2;
   h->work_ldf = w.work_ldf;
   h->y_ldf = w.y_ldf;
 }
@@ -62,12 +63,13 @@ int ensure_work(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
 int ensure_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
   auto& w = workset(h, s);
   if (w.y_ldf < ldf) {
-    for (void* p : {(void*)w.y0, (void*)w.y1})
+    for (void* p : {(void*)w.y0, (void*)w.y1, (void*)w.y2})
       if (p) h->retired.push_back(p);
-    w.y0 = w.y1 = nullptr;
+    w.y0 = w.y1 = w.y2 = nullptr;
     const size_t bytes = sizeof(double) * ((size_t)h->np << ldf);
     int rc = dalloc((void**)&w.y0, bytes);
     if (!rc) rc = dalloc((void**)&w.y1, bytes);
+    if (!rc) rc = dalloc((void**)&w.y2, bytes);
     if (rc) return rc;
     w.y_ldf = ldf;
   }
@@ -318,7 +320,7 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_chunks);
   dfree(h->d_mat_chunk0);
   for (auto& w : h->sets)
-    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1}) dfree(p);
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
   for (void* p : h->retired) dfree(p);
   delete h;
   return OCTHIP_SUCCESS;
@@ -373,10 +375,10 @@ extern "C" int octhip_hamiltonian_release_stream(octhip_hamiltonian_t h, void* s
     if (h->sets[i].s != s) continue;
     OCTHIP_HIP(hipStreamSynchronize(s));
     const auto w = h->sets[i];
-    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1}) dfree(p);
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
     h->sets.erase(h->sets.begin() + (long)i);
     // the mirrored pointers may name the freed set: they are rebound by the next call
-    h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = nullptr;
+    h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = h->d_y[2] = nullptr;
     h->work_ldf = h->y_ldf = -1;
     break;
   }
@@ -592,6 +594,49 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
     OCTHIP_HIP(hipMemcpyAsync(psi, psi_in, sizeof(double) * op->np * L, hipMemcpyDeviceToDevice, s));
   std::complex<double> zf[33];
   taylor_coefs(deltat, imag_time, order, zf);
+  if (taylor_deferred(order)) {
+    // y_k = H y_{k-1} (y_0 = psi) stored for k < order (save[k] when saving); the last
+    // pass forms psi + z_1 y_1 + ... + z_order H y_{order-1}, adding the terms in order k,
+    // as exponential.F90:733-744 accumulates them order by order.  No accumulator is
+    // read and written at the orders in between.
+    auto Y = [&](int k) -> double* { return save ? save[k] : h->d_y[k - 1]; };
+    for (int k = 1; k <= order; ++k) {
+      const double* yin = (k == 1) ? src : Y(k - 1);
+      rc = run_bra(h, yin, L, s, (k == 1 && h->nmat > 0 && !psi_in) ? psi : nullptr);
+      if (rc) return rc;
+      ApplyArgs a;
+      std::memset(&a, 0, sizeof(a));
+      fill_terms(h, OCTHIP_TERM_ALL, a);
+      a.fi = yin;
+      a.L = L;
+      a.lim = op->np;  // boundary rows are zero: psi's were just cleared, work buffers have none
+      if (k < order) {
+        a.fo = Y(k);
+        rc = launch_operator(h->kin, a, EPI_STORE, s);
+      } else {
+        a.acc = psi;
+        a.acc_in = psi_in;  // null: in place
+        if (order >= 3) {
+          a.ya = Y(1);
+          a.car = zf[1].real();
+          a.cai = zf[1].imag();
+        }
+        if (order >= 4) {
+          a.yb = Y(2);
+          a.cbr = zf[2].real();
+          a.cbi = zf[2].imag();
+        }
+        a.has_c1 = 1;  // the pass's own input y_{order-1}
+        a.c1r = zf[order - 1].real();
+        a.c1i = zf[order - 1].imag();
+        a.c2r = zf[order].real();
+        a.c2i = zf[order].imag();
+        rc = launch_operator(h->kin, a, EPI_TAYLOR, s);
+      }
+      if (rc) return rc;
+    }
+    return OCTHIP_SUCCESS;
+  }
   std::complex<double> z, z1;
   for (int k = 1; k <= order; ++k) {
     z = zf[k];

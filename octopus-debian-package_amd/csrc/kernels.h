@@ -41,13 +41,17 @@ struct ApplyArgs {
   const int* __restrict__ kslot;
   const double* __restrict__ ksum;
   int nk;                          // rows of ksum
-  // Taylor epilogue: yout = Hy ; acc = acc_in + c1 * y + c2 * Hy (complex lanes), acc_in
-  // = acc when null (in place)
+  // Taylor epilogue: yout = Hy ; acc = acc_in + ca * ya + cb * yb + c1 * y + c2 * Hy
+  // (complex lanes, added in this order; acc_in = acc when null, i.e. in place; ya, yb
+  // optional: earlier powers added at the last order)
   double* yout;
   double* acc;
   const double* acc_in;
   double c1r, c1i, c2r, c2i;
   int has_c1;
+  const double* ya;
+  const double* yb;
+  double car, cai, cbr, cbi;
 };
 
 __device__ __forceinline__ int tile_of(int b, int chunk) {
@@ -199,6 +203,16 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const ApplyArgs a) {
         if (a.yout) stv<LPT>(a.yout + (size_t)p * L + l, s);
         if (a.acc) {
           Vec<LPT> av = ldv<LPT>((a.acc_in ? a.acc_in : a.acc) + (size_t)p * L + l);
+          if (a.ya) {
+            const Vec<LPT> u = ldv<LPT>(a.ya + (size_t)p * L + l);
+            av.v[0] = av.v[0] + zmul_lane(a.car, a.cai, u.v[0], u.v[1], false);
+            av.v[1] = av.v[1] + zmul_lane(a.car, a.cai, u.v[1], u.v[0], true);
+          }
+          if (a.yb) {
+            const Vec<LPT> u = ldv<LPT>(a.yb + (size_t)p * L + l);
+            av.v[0] = av.v[0] + zmul_lane(a.cbr, a.cbi, u.v[0], u.v[1], false);
+            av.v[1] = av.v[1] + zmul_lane(a.cbr, a.cbi, u.v[1], u.v[0], true);
+          }
           if (a.has_c1) {
             av.v[0] = av.v[0] + zmul_lane(a.c1r, a.c1i, x.v[0], x.v[1], false);
             av.v[1] = av.v[1] + zmul_lane(a.c1r, a.c1i, x.v[1], x.v[0], true);

@@ -178,7 +178,7 @@ struct octhip_hamiltonian_s {
   struct WorkSet {
     hipStream_t s;
     int work_ldf, y_ldf;
-    double *partial, *proj, *ksum, *y0, *y1;
+    double *partial, *proj, *ksum, *y0, *y1, *y2;
   };
   std::vector<WorkSet> sets;
   long graph_sets = 0;
@@ -186,7 +186,7 @@ struct octhip_hamiltonian_s {
   double* d_partial = nullptr;  // nchunk * max_nprojs * L
   double* d_proj = nullptr;     // full_projection_size * L
   double* d_ksum = nullptr;     // (total_points + nmulti) * L ket rows
-  double* d_y[2] = {nullptr, nullptr};  // Taylor intermediates, np rows each
+  double* d_y[3] = {nullptr, nullptr, nullptr};  // Taylor powers H^k psi, np rows each
   int y_ldf = -1;
   std::vector<void*> retired;   // work buffers outgrown by a larger ldf, freed at destroy
   int use_mfma = 1;
@@ -195,6 +195,9 @@ struct octhip_hamiltonian_s {
 namespace octhip {
 // Taylor building blocks shared by the propagator entry points and the adjoint.
 int prepare_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s);
+// Orders 2..4 keep every power H^k psi (k < order) and add them all at the last order;
+// higher orders accumulate into psi order by order.
+inline bool taylor_deferred(int order) { return order >= 2 && order <= 4; }
 // psi_in (optional): take psi_s from psi_in and write psi_{s+1} to the np inner rows of psi
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
                 double* const* save, hipStream_t s, const double* psi_in = nullptr);

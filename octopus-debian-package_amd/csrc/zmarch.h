@@ -125,6 +125,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   double* const a_acc = A.acc;
   const double a_c1r = A.c1r, a_c1i = A.c1i, a_c2r = A.c2r, a_c2i = A.c2i;
   const int a_has_c1 = A.has_c1;
+  const double a_car = A.car, a_cai = A.cai, a_cbr = A.cbr, a_cbi = A.cbi;
   double a_w[25];
 #pragma unroll
   for (int j = 0; j < 25; ++j) a_w[j] = A.w[j];
@@ -144,6 +145,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const auto r_fo = zrsrc(A.fo, EPI == EPI_STORE ? np_bytes : 0u);
   const auto r_y = zrsrc(A.yout, A.yout ? np_bytes : 0u);
   const auto r_acc = zrsrc(a_acc, has_acc ? np_bytes : 0u);
+  const auto r_ya = zrsrc(A.ya, (has_acc && A.ya) ? np_bytes : 0u);  // earlier powers, last order
+  const auto r_yb = zrsrc(A.yb, (has_acc && A.yb) ? np_bytes : 0u);
   auto roff = [&](int q) { return ((uint32_t)q << sh) + lbyte; };
   // halo units of this thread: halo site h = col + ZNC*i (if h < ZNH), same lane pair
   int hlds[ZHU];
@@ -230,6 +233,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   double vr[AR] = {};
   Vec<LPT> kr[AR];
   Vec<LPT> ar[AR];
+  Vec<LPT> ra[EPI == EPI_TAYLOR ? AR : 1], rb[EPI == EPI_TAYLOR ? AR : 1];
   // Every global load below is issued unconditionally (out-of-range requests read zeros):
   // with loads behind branches the compiler cannot count the loads still in flight and
   // drains the whole prefetch queue at every plane.
@@ -243,7 +247,11 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       const int ks = tks[(P - zb + 4) * ZNC + col];
       kr[slot] = bld<LPT>(r_k, roff(qo >= 0 ? ks : -1));
     }
-    if constexpr (EPI == EPI_TAYLOR) ar[slot] = bld<LPT>(r_ai, roff(qo));
+    if constexpr (EPI == EPI_TAYLOR) {
+      ar[slot] = bld<LPT>(r_ai, roff(qo));
+      ra[slot] = bld<LPT>(r_ya, roff(qo));  // zeros (no memory access) without ya / yb
+      rb[slot] = bld<LPT>(r_yb, roff(qo));
+    }
   };
 #pragma unroll
   for (int k = 0; k < OA + 4; ++k) ov[k] = val(zb - 4 + k < ze + 4 ? own_idx(zb - 4 + k) : -1);
@@ -331,7 +339,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
           bst<LPT>(r_y, roff(q0), s);  // dropped when there is no yout
           if (has_acc) {
             Vec<LPT> av = ar[kk % AR];
+            const Vec<LPT> ua = ra[kk % AR], ub = rb[kk % AR];
             if constexpr (LPT == 2) {
+              av.v[0] = av.v[0] + zmul_lane(a_car, a_cai, ua.v[0], ua.v[1], false);
+              av.v[1] = av.v[1] + zmul_lane(a_car, a_cai, ua.v[1], ua.v[0], true);
+              av.v[0] = av.v[0] + zmul_lane(a_cbr, a_cbi, ub.v[0], ub.v[1], false);
+              av.v[1] = av.v[1] + zmul_lane(a_cbr, a_cbi, ub.v[1], ub.v[0], true);
               if (a_has_c1) {
                 av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], x.v[1], false);
                 av.v[1] = av.v[1] + zmul_lane(a_c1r, a_c1i, x.v[1], x.v[0], true);
@@ -341,6 +354,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
             } else {
               // the partner lane of a (re, im) pair is held by the neighbouring thread
               const bool odd = (l & 1) != 0;
+              av.v[0] = av.v[0] + zmul_lane(a_car, a_cai, ua.v[0], __shfl_xor(ua.v[0], 1), odd);
+              av.v[0] = av.v[0] + zmul_lane(a_cbr, a_cbi, ub.v[0], __shfl_xor(ub.v[0], 1), odd);
               if (a_has_c1) av.v[0] = av.v[0] + zmul_lane(a_c1r, a_c1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
               av.v[0] = av.v[0] + zmul_lane(a_c2r, a_c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
             }
