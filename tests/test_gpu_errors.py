@@ -68,6 +68,11 @@ def test_tuning_rejects_oversized_lds_and_bad_values(gpu):
     gpu.cuda.synchronize()
 
 
+def test_release_stream_rejects_null_handle(gpu):
+    with pytest.raises(_lib.OcthipError):
+        _lib.call("octhip_hamiltonian_release_stream", None, None)
+
+
 def test_adjoint_backward_needs_forward(gpu):
     s = small_sphere(radius=2.5, spacing=0.3)
     hm = api.HamiltonianDevice.from_system(s)
