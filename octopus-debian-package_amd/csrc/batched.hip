@@ -43,10 +43,12 @@ struct Pair {
   double* proj = nullptr;
   double* ksum = nullptr;
   double* y[3] = {nullptr, nullptr, nullptr};  // Taylor powers (three for order 4)
-  const int4* items = nullptr;   // the operator's z-march items for this pair's segment length
-  const int* tab = nullptr;      // and their index tables
-  const int* tab_off = nullptr;
-  int nitems = 0;
+  // the operator's z-march items and their index tables, one plan per epilogue ([0] plain
+  // store, [1] Taylor): the two instances keep different numbers of workgroups resident
+  const int4* items[2] = {nullptr, nullptr};
+  const int* tab[2] = {nullptr, nullptr};
+  const int* tab_off[2] = {nullptr, nullptr};
+  int nitems[2] = {0, 0};
 };
 
 // a device table of argument blocks plus block offsets for one multi-segment kernel
@@ -82,13 +84,14 @@ struct octhip_batched_s {
   int order = 4;
   double dt = 0.0;
   bool mfma = true, nl = false;
-  int lc = 16, lpt = 2, zseg = 8;
+  int lc = 16, lpt = 2;
+  int zseg[2] = {8, 8};                 // per plan: longest segment (LDS size)
   std::complex<double> zf[33];
   Seg zero, reduce, ket, ksum;
   std::vector<Seg> bra;                 // [0] H-apply input psi, [k] Taylor order k input
   std::vector<ZArgs*> zdesc;            // [0] H-apply (store to hpsi), [k] Taylor order k
-  int4* sched = nullptr;
-  int sched_len = 0;
+  int4* sched[2] = {nullptr, nullptr};  // per plan: blockIdx -> (pair, item, lane chunk)
+  int sched_len[2] = {0, 0};
 };
 
 namespace {
@@ -108,7 +111,8 @@ void destroy(octhip_batched_s* b) {
   free_seg(b->ksum);
   for (auto& s : b->bra) free_seg(s);
   for (auto* z : b->zdesc) dfree(z);
-  dfree(b->sched);
+  dfree(b->sched[0]);
+  dfree(b->sched[1]);
   delete b;
 }
 
@@ -149,6 +153,12 @@ int build_bra(octhip_batched_s* b, const std::vector<const double*>& in, Seg& ou
 const double* pass_input(const Pair& p, int k, int order) {
   if (k <= 1) return p.psi;
   return taylor_deferred(order) ? p.y[k - 2] : p.y[(k - 2) & 1];
+}
+
+// epilogue of pass k (0 = the H-apply), which is also its plan index
+int pass_epi(const octhip_batched_s* b, int k) {
+  if (k == 0) return EPI_STORE;
+  return (taylor_deferred(b->order) && k < b->order) ? EPI_STORE : EPI_TAYLOR;
 }
 
 // z-march descriptors of every pair for one pass
@@ -197,9 +207,10 @@ int build_zdesc(octhip_batched_s* b, int k, ZArgs** out) {
         z.a.c2i = b->zf[k].imag();
       }
     }
-    z.items = p.items;
-    z.tab = p.tab;
-    z.tab_off = p.tab_off;
+    const int plan = pass_epi(b, k);
+    z.items = p.items[plan];
+    z.tab = p.tab[plan];
+    z.tab_off = p.tab_off[plan];
     z.sched = nullptr;
     // every pointer the kernel dereferences must exist
     // (the H-apply descriptor may lack fo: octhip_batched_apply refuses to run without hpsi)
@@ -224,10 +235,12 @@ int run_bra(octhip_batched_s* b, const Seg& bra, hipStream_t s) {
   return rc;
 }
 
-int run_pass(octhip_batched_s* b, int k, int epi, hipStream_t s) {
+int run_pass(octhip_batched_s* b, int k, hipStream_t s) {
   int rc = run_bra(b, b->bra[k], s);
   if (rc) return rc;
-  return launch_zmarch_multi(b->zdesc[k], b->sched, b->sched_len, b->lc, b->lpt, epi, b->nl, b->zseg, s);
+  const int epi = pass_epi(b, k);
+  return launch_zmarch_multi(b->zdesc[k], b->sched[epi], b->sched_len[epi], b->lc, b->lpt, epi, b->nl,
+                             b->zseg[epi], s);
 }
 
 }  // namespace
@@ -263,8 +276,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   b->mfma = minL >= 16;
   for (int i = 0; i < npairs; ++i) b->mfma = b->mfma && hms[i]->use_mfma;
   taylor_coefs(deltat, 0, order, b->zf);
-  const int resident = std::min(zmarch_reg_blocks(b->lc, b->lpt, EPI_STORE, b->nl, true),
-                                zmarch_reg_blocks(b->lc, b->lpt, EPI_TAYLOR, b->nl, true));
+  const int resident[2] = {zmarch_reg_blocks(b->lc, b->lpt, EPI_STORE, b->nl, true),
+                           zmarch_reg_blocks(b->lc, b->lpt, EPI_TAYLOR, b->nl, true)};
   int rc = OCTHIP_SUCCESS;
   // pairs and their work buffers
   for (int i = 0; i < npairs && !rc; ++i) {
@@ -283,14 +296,16 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     if (!rc) rc = dalloc((void**)&p.y[0], sizeof(double) * h->np * L);
     if (!rc) rc = dalloc((void**)&p.y[1], sizeof(double) * h->np * L);
     if (!rc && taylor_deferred(order) && order == 4) rc = dalloc((void**)&p.y[2], sizeof(double) * h->np * L);
-    // the items serve both epilogues: segments for the residency both instances allow
-    if (!rc) rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident));
-    if (!rc) {
-      p.items = h->kin->d_items;
-      p.tab = h->kin->d_tab;
-      p.tab_off = h->kin->d_tab_off;
-      p.nitems = h->kin->nitems;
-      b->zseg = std::max(b->zseg, h->kin->zseg_built);
+    // one plan per epilogue, segments for the residency of that instance
+    for (int plan = 0; plan < 2 && !rc; ++plan) {
+      rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident[plan]));
+      if (!rc) {
+        p.items[plan] = h->kin->d_items;
+        p.tab[plan] = h->kin->d_tab;
+        p.tab_off[plan] = h->kin->d_tab_off;
+        p.nitems[plan] = h->kin->nitems;
+        b->zseg[plan] = std::max(b->zseg[plan], h->kin->zseg_built);
+      }
     }
     b->pairs.push_back(p);
   }
@@ -354,13 +369,14 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     if (!rc) rc = build_zdesc(b, k, &zd);
     b->zdesc.push_back(zd);
   }
-  // (pair, item, lane chunk) work list, cut into 8 contiguous parts dealt to the 8 XCDs
-  if (!rc) {
+  // (pair, item, lane chunk) work list of each plan, cut into 8 contiguous parts dealt to
+  // the 8 XCDs
+  for (int plan = 0; plan < 2 && !rc; ++plan) {
     std::vector<int4> all;
     for (size_t i = 0; i < b->pairs.size(); ++i) {
       const int nchunk = b->pairs[i].L / b->lc;
       for (int c = 0; c < nchunk; ++c)
-        for (int it = 0; it < b->pairs[i].nitems; ++it) all.push_back(make_int4((int)i, it, c, 0));
+        for (int it = 0; it < b->pairs[i].nitems[plan]; ++it) all.push_back(make_int4((int)i, it, c, 0));
     }
     const int n = (int)all.size();
     const int per = (n + 7) / 8;
@@ -371,8 +387,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
         if (g >= n) break;
         sc[(size_t)i * 8 + k] = all[g];
       }
-    rc = upload(&b->sched, sc.data(), sc.size(), nullptr);
-    b->sched_len = (int)sc.size();
+    rc = upload(&b->sched[plan], sc.data(), sc.size(), nullptr);
+    b->sched_len[plan] = (int)sc.size();
   }
   if (!rc) {
     hipError_t e = hipDeviceSynchronize();
@@ -393,7 +409,7 @@ extern "C" int octhip_batched_apply(octhip_batched_t b, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_multi(MULTI_ZERO, b->zero.args, b->zero.off, b->zero.nseg, b->zero.nblocks, s);
   if (rc) return rc;
-  return run_pass(b, 0, EPI_STORE, s);
+  return run_pass(b, 0, s);
 }
 
 extern "C" int octhip_batched_taylor_step(octhip_batched_t b, void* stream) {
@@ -401,9 +417,7 @@ extern "C" int octhip_batched_taylor_step(octhip_batched_t b, void* stream) {
   OCTHIP_REQUIRE(b != nullptr, "null handle");
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_multi(MULTI_ZERO, b->zero.args, b->zero.off, b->zero.nseg, b->zero.nblocks, s);
-  const bool deferred = taylor_deferred(b->order);
-  for (int k = 1; k <= b->order && !rc; ++k)
-    rc = run_pass(b, k, (deferred && k < b->order) ? EPI_STORE : EPI_TAYLOR, s);
+  for (int k = 1; k <= b->order && !rc; ++k) rc = run_pass(b, k, s);
   return rc;
 }
 

@@ -109,7 +109,9 @@ struct ZArgs {
 // One work item: tile it = {tile x, tile y, first plane, end plane}, lane chunk `chunk`.
 // za may live in kernel arguments (single operator) or in global memory (one descriptor
 // per system of a batched launch): every field is copied to a local once.
-template <int LPT, int LC, int EPI, bool NL>
+// SHALLOW: the shallow prefetch rings whatever the epilogue (the batched instances, whose
+// descriptor handling costs a few registers more).
+template <int LPT, int LC, int EPI, bool NL, bool SHALLOW = false>
 __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, const int item, const int chunk) {
   constexpr int PAIRS = LC / LPT;    // threads per column
   constexpr int NT = ZNC * PAIRS;
@@ -223,8 +225,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   //   vr/kr/ar[(P - zb) % AR] output point data,     P in z .. z+AD (loaded AD planes ahead)
   // The load counter retires in order, so the shortest distance sets how many planes of
   // loads stay in flight: halo and point data go as deep as the own values allow.
-  constexpr int RQ = ZRings<EPI>::ring, OA = ZRings<EPI>::own;
-  constexpr int HD = ZRings<EPI>::halo, AD = ZRings<EPI>::aux;
+  using Rings = ZRings<SHALLOW ? EPI_TAYLOR : EPI>;
+  constexpr int RQ = Rings::ring, OA = Rings::own;
+  constexpr int HD = Rings::halo, AD = Rings::aux;
   constexpr int HR = ring_above(RQ, HD);  // ring sizes dividing RQ
   constexpr int AR = ring_above(RQ, AD);
   static_assert(OA >= 5 && RQ >= OA + 5 && HR > HD && AR > AD, "ring sizes");
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(ZNC * LC / LPT) void k_zmarch_multi(const ZArgs* __
   const int4 sc = sched[blockIdx.x];
   if (sc.x < 0) return;
   const ZArgs& za = zs[sc.x];
-  zmarch_body<LPT, LC, EPI, NL>(za, za.items[sc.y], sc.y, sc.z);
+  zmarch_body<LPT, LC, EPI, NL, true>(za, za.items[sc.y], sc.y, sc.z);
 }
 
 // dynamic LDS bytes of k_zmarch for a lane chunk and segment length
