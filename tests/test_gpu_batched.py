@@ -93,3 +93,22 @@ def test_batched_graph_replays_bitwise(gpu):
         gpu.cuda.synchronize()
         for (_, a), (_, b) in zip(pairs, twin):
             assert gpu.equal(a.ff[: a.nrows], b.ff[: b.nrows]), f"replay {rep}"
+
+
+@pytest.mark.parametrize("order", [2, 3, 6])
+def test_batched_other_orders_match_per_pair_calls(gpu, order):
+    """Orders 2 and 3 (powers added in the last pass, fewer of them) and 6 (accumulated
+    order by order) against per-pair Taylor steps of the same order."""
+    print(f"[batched order {order}]", flush=True)
+    pairs, _ = make_pairs(systems()[:2])
+    eager = [(hm, b.clone()) for hm, b in pairs]
+    bp = api.BatchedPropagator(pairs, 0.01, order=order, with_apply=False)
+    te = api.Exponential(order)
+    for step in range(2):
+        bp.taylor_step()
+        for hm, b in eager:
+            te.apply_batch(hm, b, 0.01)
+        gpu.cuda.synchronize()
+        for (_, b), (_, e) in zip(pairs, eager):
+            assert gpu.equal(b.ff[: b.nrows], e.ff[: e.nrows]), f"order {order} step {step}"
+

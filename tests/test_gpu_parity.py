@@ -168,7 +168,7 @@ def test_taylor_step_workload_batch(gpu):
     assert max_rel_err(psib.ff.cpu().numpy()[: s.np], psi[: s.np]) <= 1e-12
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 6])
 @pytest.mark.parametrize("imag_time", [False, True])
 @pytest.mark.parametrize("kernel,lpt", [(1, 0), (2, 2), (2, 1)])
 def test_taylor_step(gpu, order, imag_time, kernel, lpt):
