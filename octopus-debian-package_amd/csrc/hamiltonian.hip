@@ -484,21 +484,52 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
   *out = nullptr;
+  // Batches alternate between the capturing stream and a second branch of the graph
+  // (forked and joined with events), as the reference gives every batch its own stream
+  // (src/td/propagation_ops_elec.F90:236): the kernels of neighbouring batches overlap,
+  // each branch with its own work buffers.
+  const bool two = nbatch > 1;
+  hipStream_t s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  auto cleanup = [&]() {
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (s2) (void)hipStreamDestroy(s2);
+  };
+  if (two && (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess)) {
+    cleanup();
+    return fail(OCTHIP_ERR_HIP, "taylor_graph_create: stream or event creation failed");
+  }
   hipStream_t used = nullptr;
   int rc = capture_graph((hipStream_t)stream,
                        [&](hipStream_t cs) {
                          used = cs;
-                         return prepare_taylor(h, ldf, cs);
+                         int r = prepare_taylor(h, ldf, cs);
+                         if (!r && two) r = prepare_taylor(h, ldf, s2);
+                         return r;
                        },
                        [&](hipStream_t cs) {
+                         if (two) {
+                           OCTHIP_HIP(hipEventRecord(fork, cs));
+                           OCTHIP_HIP(hipStreamWaitEvent(s2, fork, 0));
+                         }
                          int r = OCTHIP_SUCCESS;
                          for (int b = 0; b < nbatch && !r; ++b)
-                           r = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr, cs, nullptr);
+                           r = taylor_step(h, psi[b], ldf, deltat, order, imag_time, nullptr,
+                                           (two && (b & 1)) ? s2 : cs, nullptr);
+                         if (two) {
+                           OCTHIP_HIP(hipEventRecord(join, s2));
+                           OCTHIP_HIP(hipStreamWaitEvent(cs, join, 0));
+                         }
                          return r;
                        },
                        out);
   // the graph owns the buffers it was captured with: later calls on any stream get others
   pin_workset(h, used);
+  if (two) pin_workset(h, s2);
+  cleanup();
   return rc;
 }
 
