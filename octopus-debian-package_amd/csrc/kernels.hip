@@ -190,12 +190,14 @@ __device__ __forceinline__ void bra_reduce_body(const BraReduceArgs& r, int bid,
     const int j = i - r.mat_pbase[a];
     double s = 0.0;
     const int c0 = r.mat_chunk0[a], c1 = r.mat_chunk0[a + 1];
-    for (int c = c0; c < c1; c += 8) {
-      double v[8];
+    // a matrix's partials (one per chunk of <= 256 sphere points: 29 for a radius-3
+    // sphere at h = 0.25) are loaded together, clamped indices, then added in chunk order
+    for (int c = c0; c < c1; c += 32) {
+      double v[32];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = (c + u < c1) ? r.partial[((size_t)(c + u) * r.pjmax + j) * r.L + l] : 0.0;
+      for (int u = 0; u < 32; ++u) v[u] = r.partial[((size_t)min(c + u, c1 - 1) * r.pjmax + j) * r.L + l];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 32; ++u)
         if (c + u < c1) s += v[u];
     }
     r.proj[t] = s * r.scals[i];
