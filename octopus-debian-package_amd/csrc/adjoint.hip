@@ -149,13 +149,16 @@ struct octhip_adjoint_s {
   double dt = 0.0, vol = 0.0;
   double pol[3] = {0.0, 0.0, 0.0};
   double* d_xp = nullptr;       // np
-  double* d_slot = nullptr;     // (K+1) x np x L: psi_s of one segment (recomputed)
+  double* d_slot = nullptr;     // 2 x (K+1) x np x L: psi_s of a segment (recomputed), two sets
   double* d_ckpt = nullptr;     // nckpt x np x L: psi at every K-th step
-  double* d_seg = nullptr;      // K x (order-1) x np x L: H^k psi_s, k >= 1, of one segment
+  double* d_seg = nullptr;      // 2 x K x (order-1) x np x L: H^k psi_s, k >= 1, two sets
   double* d_ychi = nullptr;     // order x np x L: H^m chi_{s+1}, m >= 1 (slot 0 unused)
   double* d_chi2 = nullptr;     // np x L: the other chi buffer of the backward ping-pong
   double* d_part = nullptr;     // ADJ_GRID
   double* d_deps = nullptr;     // nsteps
+  double* d_veff = nullptr;     // 2 x np: v + eps_s x of the recompute and the backward stream
+  hipStream_t aux = nullptr;    // recompute stream of the backward
+  hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr};
   int nckpt = 0;
   std::vector<double> eps;      // field of every step (set by forward)
   bool forward_done = false;
@@ -172,6 +175,13 @@ void free_adjoint(octhip_adjoint_s* a) {
   dfree(a->d_ychi);
   dfree(a->d_part);
   dfree(a->d_deps);
+  dfree(a->d_veff);
+  for (hipEvent_t e : {a->ev_fork, a->ev_rec[0], a->ev_rec[1], a->ev_back[0], a->ev_back[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (a->aux) {
+    (void)hipStreamSynchronize(a->aux);
+    (void)hipStreamDestroy(a->aux);
+  }
   delete a;
 }
 
@@ -215,10 +225,18 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
     xp[p] = pol[0] * positions[3 * (size_t)p] + pol[1] * positions[3 * (size_t)p + 1] +
             pol[2] * positions[3 * (size_t)p + 2];
   if (!rc) rc = upload(&a->d_xp, xp.data(), xp.size(), nullptr);
-  if (!rc) rc = dalloc((void**)&a->d_slot, rows_bytes((size_t)(a->K + 1) * np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_slot, rows_bytes((size_t)2 * (a->K + 1) * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_chi2, rows_bytes(np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_ckpt, rows_bytes((size_t)a->nckpt * np, a->L));
-  if (!rc && order > 1) rc = dalloc((void**)&a->d_seg, rows_bytes((size_t)a->K * (order - 1) * np, a->L));
+  if (!rc && order > 1) rc = dalloc((void**)&a->d_seg, rows_bytes((size_t)2 * a->K * (order - 1) * np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_veff, sizeof(double) * 2 * (size_t)np);
+  if (!rc && (hipStreamCreateWithFlags(&a->aux, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_rec[0], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_rec[1], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_back[0], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_back[1], hipEventDisableTiming) != hipSuccess))
+    rc = fail(OCTHIP_ERR_HIP, "adjoint_create: stream or event creation failed");
   if (!rc) rc = dalloc((void**)&a->d_ychi, rows_bytes((size_t)order * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_part, sizeof(double) * ADJ_GRID);
   if (!rc) rc = dalloc((void**)&a->d_deps, sizeof(double) * nsteps);
@@ -297,30 +315,68 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
   // chi ping-pong: cur holds chi_{s+1}, the step writes chi_s into oth (np inner rows)
   double* cur = chi;
   double* oth = a->d_chi2;
-  for (int c = a->nckpt - 1; c >= 0; --c) {
+  // Segment c is recomputed from its checkpoint into buffer set c & 1 on the aux stream
+  // while the backward steps of segment c + 1 run on s: recompute(c) waits for the backward
+  // of segment c + 2 (the same set) to finish reading, and the backward of c waits for
+  // recompute(c).  Each stream forms its own potential v + eps_s x (d_veff[0] recompute,
+  // d_veff[1] backward) and has its own work buffers.
+  const size_t slot_set = (size_t)(a->K + 1) * arr, seg_set = (size_t)a->K * (order - 1) * arr;
+  double* const vr = a->d_veff;
+  double* const vb = a->d_veff + np;
+  hipStream_t s2 = a->aux;
+  auto field_of = [&](int st, double f[3]) {
+    for (int d = 0; d < 3; ++d) f[d] = a->eps[st] * a->pol[d];
+  };
+  bool back_recorded[2] = {false, false};
+  auto recompute = [&](int c) -> int {
+    const int b = c & 1;
+    if (back_recorded[b]) OCTHIP_HIP(hipStreamWaitEvent(s2, a->ev_back[b], 0));
+    double* slot = a->d_slot + b * slot_set;
+    double* seg = a->d_seg + b * seg_set;
     const int s0 = c * a->K, s1 = std::min(s0 + a->K, a->nsteps);
-    // recompute the segment from its checkpoint: slot i holds psi_{s0+i}, seg the powers
-    OCTHIP_HIP(hipMemcpyAsync(a->d_slot, a->d_ckpt + (size_t)c * np * L, rows_bytes(np, L), hipMemcpyDeviceToDevice,
-                              s));
+    OCTHIP_HIP(hipMemcpyAsync(slot, a->d_ckpt + (size_t)c * np * L, rows_bytes(np, L), hipMemcpyDeviceToDevice, s2));
+    double* ys[ADJ_MAX_ORDER];
     for (int st = s0; st < s1; ++st) {
       const int i = st - s0;
-      ysave[0] = a->d_slot + (size_t)i * arr;
-      for (int k = 1; k < order; ++k) ysave[k] = a->d_seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
-      int rc = step_field(a, st, s);
-      if (!rc) rc = taylor_step(a->h, a->d_slot + (size_t)(i + 1) * arr, a->ldf, a->dt, order, 0, ysave, s, ysave[0]);
+      ys[0] = slot + (size_t)i * arr;
+      for (int k = 1; k < order; ++k) ys[k] = seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
+      double f[3];
+      field_of(st, f);
+      int rc = launch_veff(a->h, vr, f, s2);
+      if (!rc) rc = taylor_step(a->h, slot + (size_t)(i + 1) * arr, a->ldf, a->dt, order, 0, ys, s2, ys[0], vr);
       if (rc) return rc;
     }
+    OCTHIP_HIP(hipEventRecord(a->ev_rec[b], s2));
+    return OCTHIP_SUCCESS;
+  };
+  // the aux stream starts after everything the caller queued on s
+  OCTHIP_HIP(hipEventRecord(a->ev_fork, s));
+  OCTHIP_HIP(hipStreamWaitEvent(s2, a->ev_fork, 0));
+  int rc = recompute(a->nckpt - 1);
+  if (rc) return rc;
+  for (int c = a->nckpt - 1; c >= 0; --c) {
+    const int b = c & 1;
+    if (c >= 1) {
+      rc = recompute(c - 1);
+      if (rc) return rc;
+    }
+    OCTHIP_HIP(hipStreamWaitEvent(s, a->ev_rec[b], 0));
+    double* slot = a->d_slot + b * slot_set;
+    double* seg = a->d_seg + b * seg_set;
+    const int s0 = c * a->K, s1 = std::min(s0 + a->K, a->nsteps);
     // backward through the segment
     for (int st = s1 - 1; st >= s0; --st) {
       const int i = st - s0;
-      acc.yp[0] = a->d_slot + (size_t)i * arr;
-      for (int k = 1; k < order; ++k) acc.yp[k] = a->d_seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
+      acc.yp[0] = slot + (size_t)i * arr;
+      for (int k = 1; k < order; ++k) acc.yp[k] = seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
       ysave[0] = cur;
       for (int m = 1; m < order; ++m) ysave[m] = a->d_ychi + m * arr;
       acc.yc[0] = cur;
-      int rc = step_field(a, st, s);
+      double f[3];
+      field_of(st, f);
+      rc = launch_veff(a->h, vb, f, s);
       // chi_s = p(+i dt H_s) chi_{s+1}, keeping H^m chi_{s+1}
-      if (!rc) rc = taylor_step(a->h, oth, a->ldf, -a->dt, order, 0, ysave, s, cur);
+      if (!rc) rc = taylor_step(a->h, oth, a->ldf, -a->dt, order, 0, ysave, s, cur, vb);
       if (rc) return rc;
       {
         ProfScope prof("k_adj_accum", s);
@@ -332,6 +388,8 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
       OCTHIP_HIP(hipGetLastError());
       std::swap(cur, oth);
     }
+    OCTHIP_HIP(hipEventRecord(a->ev_back[b], s));
+    back_recorded[b] = true;
   }
   // chi_0 into the caller's buffer, boundary rows zero
   if (cur != chi) OCTHIP_HIP(hipMemcpyAsync(chi, cur, rows_bytes(np, L), hipMemcpyDeviceToDevice, s));

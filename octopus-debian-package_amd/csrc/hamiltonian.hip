@@ -139,6 +139,23 @@ __global__ void k_veff(double* __restrict__ veff, const double* __restrict__ v,
     veff[p] = v[p] + (f0 * xyz[3 * (size_t)p] + f1 * xyz[3 * (size_t)p + 1] + f2 * xyz[3 * (size_t)p + 2]);
 }
 
+}  // namespace
+
+namespace octhip {
+// out = v + field . x over the np points (a caller's buffer, e.g. one per stream)
+int launch_veff(const octhip_hamiltonian_s* h, double* out, const double field[3], hipStream_t s) {
+  if (!h->d_v || !h->d_xyz) return fail(OCTHIP_ERR_STATE, "launch_veff: no local potential or positions");
+  const int grid = std::min((h->np + BLOCK - 1) / BLOCK, 4096);
+  hipLaunchKernelGGL(k_veff, dim3(grid), dim3(BLOCK), 0, s, out, h->d_v, h->d_xyz, field[0], field[1], field[2],
+                     h->np);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_veff: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+}  // namespace octhip
+
+namespace {
+
 int update_veff(octhip_hamiltonian_s* h, hipStream_t s) {
   if (!h->has_field || !h->d_v || !h->d_xyz) return OCTHIP_SUCCESS;
   if (!h->d_veff) {
@@ -610,7 +627,7 @@ namespace octhip {
 // save[0] = psi (boundary cleared) and save[k] = H^k psi for k = 1..order-1: the terms the
 // discrete adjoint contracts with the backward states.
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                double* const* save, hipStream_t s, const double* psi_in) {
+                double* const* save, hipStream_t s, const double* psi_in, const double* v_override) {
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
   int rc = prepare_taylor(h, ldf, s);  // the work buffers of stream s (allocated before a capture)
@@ -642,6 +659,7 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
       ApplyArgs a;
       std::memset(&a, 0, sizeof(a));
       fill_terms(h, OCTHIP_TERM_ALL, a);
+      if (v_override) a.v = v_override;
       a.fi = yin;
       a.L = L;
       a.lim = op->np;  // boundary rows are zero: psi's were just cleared, work buffers have none
@@ -682,6 +700,7 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
     ApplyArgs a;
     std::memset(&a, 0, sizeof(a));
     fill_terms(h, OCTHIP_TERM_ALL, a);
+    if (v_override) a.v = v_override;
     a.fi = yin;
     a.L = L;
     a.lim = op->np;  // boundary rows are zero: psi's were just cleared, work buffers have none

@@ -199,8 +199,12 @@ int prepare_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s);
 // higher orders accumulate into psi order by order.
 inline bool taylor_deferred(int order) { return order >= 2 && order <= 4; }
 // psi_in (optional): take psi_s from psi_in and write psi_{s+1} to the np inner rows of psi
+// v_override (optional): the local potential to use instead of the handle's (np doubles)
 int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
-                double* const* save, hipStream_t s, const double* psi_in = nullptr);
+                double* const* save, hipStream_t s, const double* psi_in = nullptr,
+                const double* v_override = nullptr);
+// out = v + field . x (np doubles), the dipole potential of one field, on stream s
+int launch_veff(const octhip_hamiltonian_s* h, double* out, const double field[3], hipStream_t s);
 void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>* z);
 // launches of body(capturing stream) as one instantiated graph (octhip_taylor_graph_launch)
 // prepare(capturing stream) runs before the capture starts (allocations go there)
