@@ -215,10 +215,12 @@ int octhip_batched_destroy(octhip_batched_t batched);
  * discrete propagator: given chi_N with dJ = 2 Re <chi_N|d psi_N> (<a|b> = vol_pp sum
  * conj(a) b), backward returns dJ/d eps_s (host, nsteps) and dJ/dv(r) (device, np) and
  * leaves chi_0 = dJ/d psi_0* in chi.  Forward stores psi every checkpoint_every steps;
- * backward recomputes each segment from its checkpoint (memory: ceil(nsteps/K) batches
- * of np_part rows plus K*order batches of np rows).
- * create uploads positions (3*np doubles, host) into the Hamiltonian; forward and
- * backward leave the Hamiltonian's dipole field at the field of the last step touched. */
+ * backward recomputes each segment from its checkpoint, the next segment on an internal
+ * second stream while the current one runs backward (memory: ceil(nsteps/K) batches of
+ * np_part rows plus 2*(K+1) + 2*K*(order-1) + order batches of np rows).
+ * create uploads positions (3*np doubles, host) into the Hamiltonian; forward leaves the
+ * Hamiltonian's dipole field at the field of its last step (backward forms its own
+ * potentials and leaves the field as it is). */
 int octhip_adjoint_create(octhip_hamiltonian_t hm, const double* positions, const double* pol,
                           int ldf, int nsteps, double deltat, int order, int checkpoint_every,
                           double vol_pp, octhip_adjoint_t* adj);
