@@ -216,8 +216,9 @@ int octhip_batched_destroy(octhip_batched_t batched);
  * conj(a) b), backward returns dJ/d eps_s (host, nsteps) and dJ/dv(r) (device, np) and
  * leaves chi_0 = dJ/d psi_0* in chi.  Forward stores psi every checkpoint_every steps;
  * backward recomputes each segment from its checkpoint, the next segment on an internal
- * second stream while the current one runs backward (memory: ceil(nsteps/K) batches of
- * np_part rows plus 2*(K+1) + 2*K*(order-1) + order batches of np rows).
+ * second stream while the current one runs backward, and accumulates the gradients on a
+ * third (memory: ceil(nsteps/K) batches of np_part rows plus 2*(K+1) + 2*K*(order-1) +
+ * 2*order + 2 batches of np rows).
  * create uploads positions (3*np doubles, host) into the Hamiltonian; forward leaves the
  * Hamiltonian's dipole field at the field of its last step (backward forms its own
  * potentials and leaves the field as it is). */

@@ -152,13 +152,15 @@ struct octhip_adjoint_s {
   double* d_slot = nullptr;     // 2 x (K+1) x np x L: psi_s of a segment (recomputed), two sets
   double* d_ckpt = nullptr;     // nckpt x np x L: psi at every K-th step
   double* d_seg = nullptr;      // 2 x K x (order-1) x np x L: H^k psi_s, k >= 1, two sets
-  double* d_ychi = nullptr;     // order x np x L: H^m chi_{s+1}, m >= 1 (slot 0 unused)
-  double* d_chi2 = nullptr;     // np x L: the other chi buffer of the backward ping-pong
+  double* d_ychi = nullptr;     // 2 x order x np x L: H^m chi_{s+1}, m >= 1 (slot 0 unused), two sets
+  double* d_chi2 = nullptr;     // 2 x np x L: the other two chi buffers of the backward rotation
   double* d_part = nullptr;     // ADJ_GRID
   double* d_deps = nullptr;     // nsteps
   double* d_veff = nullptr;     // 2 x np: v + eps_s x of the recompute and the backward stream
   hipStream_t aux = nullptr;    // recompute stream of the backward
+  hipStream_t accs = nullptr;   // gradient accumulation stream of the backward
   hipEvent_t ev_fork = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr};
+  hipEvent_t ev_chi[2] = {nullptr, nullptr}, ev_acc[2] = {nullptr, nullptr};
   int nckpt = 0;
   std::vector<double> eps;      // field of every step (set by forward)
   bool forward_done = false;
@@ -176,12 +178,14 @@ void free_adjoint(octhip_adjoint_s* a) {
   dfree(a->d_part);
   dfree(a->d_deps);
   dfree(a->d_veff);
-  for (hipEvent_t e : {a->ev_fork, a->ev_rec[0], a->ev_rec[1], a->ev_back[0], a->ev_back[1]})
+  for (hipEvent_t e : {a->ev_fork, a->ev_rec[0], a->ev_rec[1], a->ev_back[0], a->ev_back[1], a->ev_chi[0],
+                       a->ev_chi[1], a->ev_acc[0], a->ev_acc[1]})
     if (e) (void)hipEventDestroy(e);
-  if (a->aux) {
-    (void)hipStreamSynchronize(a->aux);
-    (void)hipStreamDestroy(a->aux);
-  }
+  for (hipStream_t st : {a->aux, a->accs})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   delete a;
 }
 
@@ -226,7 +230,7 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
             pol[2] * positions[3 * (size_t)p + 2];
   if (!rc) rc = upload(&a->d_xp, xp.data(), xp.size(), nullptr);
   if (!rc) rc = dalloc((void**)&a->d_slot, rows_bytes((size_t)2 * (a->K + 1) * np, a->L));
-  if (!rc) rc = dalloc((void**)&a->d_chi2, rows_bytes(np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_chi2, rows_bytes((size_t)2 * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_ckpt, rows_bytes((size_t)a->nckpt * np, a->L));
   if (!rc && order > 1) rc = dalloc((void**)&a->d_seg, rows_bytes((size_t)2 * a->K * (order - 1) * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_veff, sizeof(double) * 2 * (size_t)np);
@@ -235,9 +239,14 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
               hipEventCreateWithFlags(&a->ev_rec[0], hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&a->ev_rec[1], hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&a->ev_back[0], hipEventDisableTiming) != hipSuccess ||
-              hipEventCreateWithFlags(&a->ev_back[1], hipEventDisableTiming) != hipSuccess))
+              hipEventCreateWithFlags(&a->ev_back[1], hipEventDisableTiming) != hipSuccess ||
+              hipStreamCreateWithFlags(&a->accs, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_chi[0], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_chi[1], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_acc[0], hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&a->ev_acc[1], hipEventDisableTiming) != hipSuccess))
     rc = fail(OCTHIP_ERR_HIP, "adjoint_create: stream or event creation failed");
-  if (!rc) rc = dalloc((void**)&a->d_ychi, rows_bytes((size_t)order * np, a->L));
+  if (!rc) rc = dalloc((void**)&a->d_ychi, rows_bytes((size_t)2 * order * np, a->L));
   if (!rc) rc = dalloc((void**)&a->d_part, sizeof(double) * ADJ_GRID);
   if (!rc) rc = dalloc((void**)&a->d_deps, sizeof(double) * nsteps);
   if (!rc) rc = prepare_taylor(h, ldf, nullptr);
@@ -309,12 +318,16 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
   acc.dv = dj_dv;
   acc.part = a->d_part;
   acc.two_vol = 2.0 * a->vol;
-  for (int m = 1; m < order; ++m) acc.yc[m] = a->d_ychi + m * arr;
   double* ysave[ADJ_MAX_ORDER];
   const int tp = std::min(L / 2, 64);
-  // chi ping-pong: cur holds chi_{s+1}, the step writes chi_s into oth (np inner rows)
-  double* cur = chi;
-  double* oth = a->d_chi2;
+  // Backward step t (t = 0, 1, ... in backward order) reads chi_{s+1} from cb[t % 3] and
+  // writes chi_s into cb[(t + 1) % 3], keeping H^m chi_{s+1} in set t & 1 of d_ychi; the
+  // gradient accumulation of step t runs on the accumulation stream (in step order, so
+  // the sums are unchanged) while step t + 1 runs: step t waits only for the accumulation
+  // of step t - 2, whose chi buffer and ychi set it overwrites.
+  double* const cb[3] = {chi, a->d_chi2, a->d_chi2 + arr};
+  hipStream_t s3 = a->accs;
+  int t = 0;
   // Segment c is recomputed from its checkpoint into buffer set c & 1 on the aux stream
   // while the backward steps of segment c + 1 run on s: recompute(c) waits for the backward
   // of segment c + 2 (the same set) to finish reading, and the backward of c waits for
@@ -349,9 +362,10 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
     OCTHIP_HIP(hipEventRecord(a->ev_rec[b], s2));
     return OCTHIP_SUCCESS;
   };
-  // the aux stream starts after everything the caller queued on s
+  // the aux and accumulation streams start after everything the caller queued on s
   OCTHIP_HIP(hipEventRecord(a->ev_fork, s));
   OCTHIP_HIP(hipStreamWaitEvent(s2, a->ev_fork, 0));
+  OCTHIP_HIP(hipStreamWaitEvent(s3, a->ev_fork, 0));
   int rc = recompute(a->nckpt - 1);
   if (rc) return rc;
   for (int c = a->nckpt - 1; c >= 0; --c) {
@@ -365,32 +379,44 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
     double* seg = a->d_seg + b * seg_set;
     const int s0 = c * a->K, s1 = std::min(s0 + a->K, a->nsteps);
     // backward through the segment
-    for (int st = s1 - 1; st >= s0; --st) {
-      const int i = st - s0;
-      acc.yp[0] = slot + (size_t)i * arr;
-      for (int k = 1; k < order; ++k) acc.yp[k] = seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
+    for (int st = s1 - 1; st >= s0; --st, ++t) {
+      const int i = st - s0, u = t & 1;
+      double* const cur = cb[t % 3];
+      double* const nxt = cb[(t + 1) % 3];
+      double* const ychi = a->d_ychi + (size_t)u * order * arr;
+      if (t >= 2) OCTHIP_HIP(hipStreamWaitEvent(s, a->ev_acc[u], 0));
       ysave[0] = cur;
-      for (int m = 1; m < order; ++m) ysave[m] = a->d_ychi + m * arr;
-      acc.yc[0] = cur;
+      for (int m = 1; m < order; ++m) ysave[m] = ychi + m * arr;
       double f[3];
       field_of(st, f);
       rc = launch_veff(a->h, vb, f, s);
       // chi_s = p(+i dt H_s) chi_{s+1}, keeping H^m chi_{s+1}
-      if (!rc) rc = taylor_step(a->h, oth, a->ldf, -a->dt, order, 0, ysave, s, cur, vb);
+      if (!rc) rc = taylor_step(a->h, nxt, a->ldf, -a->dt, order, 0, ysave, s, cur, vb);
       if (rc) return rc;
+      OCTHIP_HIP(hipEventRecord(a->ev_chi[u], s));
+      OCTHIP_HIP(hipStreamWaitEvent(s3, a->ev_chi[u], 0));
+      acc.yp[0] = slot + (size_t)i * arr;
+      for (int k = 1; k < order; ++k) acc.yp[k] = seg + ((size_t)i * (order - 1) + (k - 1)) * arr;
+      acc.yc[0] = cur;
+      for (int m = 1; m < order; ++m) acc.yc[m] = ychi + m * arr;
       {
-        ProfScope prof("k_adj_accum", s);
-        rc = order == 4 ? launch_accum<4>(tp, acc, s) : launch_accum<0>(tp, acc, s);
+        ProfScope prof("k_adj_accum", s3);
+        rc = order == 4 ? launch_accum<4>(tp, acc, s3) : launch_accum<0>(tp, acc, s3);
         if (rc) return rc;
       }
-      hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, s, a->d_part, ADJ_GRID, 2.0 * a->vol,
+      hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, s3, a->d_part, ADJ_GRID, 2.0 * a->vol,
                          a->d_deps + st);
       OCTHIP_HIP(hipGetLastError());
-      std::swap(cur, oth);
+      OCTHIP_HIP(hipEventRecord(a->ev_acc[u], s3));
     }
-    OCTHIP_HIP(hipEventRecord(a->ev_back[b], s));
+    // the segment's psi arrays are free once its last accumulation is done
+    OCTHIP_HIP(hipEventRecord(a->ev_back[b], s3));
     back_recorded[b] = true;
   }
+  // everything accumulated before the results leave
+  OCTHIP_HIP(hipEventRecord(a->ev_acc[0], s3));
+  OCTHIP_HIP(hipStreamWaitEvent(s, a->ev_acc[0], 0));
+  double* const cur = cb[t % 3];
   // chi_0 into the caller's buffer, boundary rows zero
   if (cur != chi) OCTHIP_HIP(hipMemcpyAsync(chi, cur, rows_bytes(np, L), hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_fill_zero, dim3(1024), dim3(256), 0, s, chi + arr, (size_t)(np_part - np) * L);
