@@ -26,14 +26,14 @@ def problem(nst=8, nsteps=12, dt=0.01):
     return s, vol, workloads.pack(z, s.np_part), workloads.pack(tg, s.np_part), eps
 
 
-def device_run(gpu, s, vol, psi0, phi, eps, dt, K, v=None, backward=True):
+def device_run(gpu, s, vol, psi0, phi, eps, dt, K, v=None, backward=True, order=4):
     from octhip.autograd import overlap_objective
     hm = api.HamiltonianDevice.from_system(s)
     if v is not None:
         hm.set_potential(v)
     nst = psi0.shape[1] // 2
     adj = api.Adjoint(hm, s.mesh.x, [1.0, 0.0, 0.0], int(np.log2(psi0.shape[1])), len(eps), dt, vol,
-                      checkpoint_every=K)
+                      order=order, checkpoint_every=K)
     psib = api.Batch(nst, s.np_part, True, ff=gpu.from_numpy(psi0.copy()).cuda())
     phib = api.Batch(nst, s.np_part, True, ff=gpu.from_numpy(phi.copy()).cuda())
     adj.forward(psib, eps)
@@ -49,14 +49,18 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-@pytest.mark.parametrize("K", [5, 12])
-def test_adjoint_matches_oracle(gpu, K):
-    print(f"[adjoint vs oracle K={K}]", flush=True)
-    s, vol, psi0, phi, eps = problem()
+# nsteps 12, 10 and 11 leave chi_0 in each of the three buffers the backward rotates through
+# (the caller's, then the two internal ones copied back); orders other than 4 take the
+# generic accumulation kernel
+@pytest.mark.parametrize("K,nsteps,order", [(5, 12, 4), (12, 12, 4), (4, 10, 4), (3, 11, 4), (4, 10, 2),
+                                            (5, 11, 3), (4, 7, 6)])
+def test_adjoint_matches_oracle(gpu, K, nsteps, order):
+    print(f"[adjoint vs oracle K={K} nsteps={nsteps} order={order}]", flush=True)
+    s, vol, psi0, phi, eps = problem(nsteps=nsteps)
     dt = 0.01
-    J, psiN, chi0, deps, dv = device_run(gpu, s, vol, psi0, phi, eps, dt, K)
+    J, psiN, chi0, deps, dv = device_run(gpu, s, vol, psi0, phi, eps, dt, K, order=order)
     h = O.Hamiltonian(s.op, s.v.copy(), s.proj, xyz=s.mesh.x[: s.np], field=[0.0, 0.0, 0.0])
-    oa = O.TaylorAdjoint(h, s.mesh.x[: s.np], [1.0, 0.0, 0.0], dt, vol=vol)
+    oa = O.TaylorAdjoint(h, s.mesh.x[: s.np], [1.0, 0.0, 0.0], dt, order=order, vol=vol)
     psiN_ref = oa.forward(psi0, eps)
     J_ref, chiN_ref = O.overlap_objective(phi, psiN_ref, s.np, vol)
     chi0_ref, deps_ref, dv_ref = oa.backward(chiN_ref)
@@ -74,6 +78,28 @@ def test_checkpoint_interval_does_not_change_gradients(gpu):
     for o in out[1:]:
         assert np.array_equal(o[3], out[0][3]) and np.array_equal(o[4], out[0][4])
         assert np.array_equal(o[2], out[0][2])
+
+
+def test_backward_twice_gives_the_same_gradients(gpu):
+    """A second backward on the same adjoint (same forward) reuses its streams, events and
+    buffers and must reproduce the first bit for bit."""
+    print("[adjoint backward twice]", flush=True)
+    from octhip.autograd import overlap_objective
+    s, vol, psi0, phi, eps = problem(nsteps=11)
+    hm = api.HamiltonianDevice.from_system(s)
+    adj = api.Adjoint(hm, s.mesh.x, [1.0, 0.0, 0.0], int(np.log2(psi0.shape[1])), len(eps), 0.01, vol,
+                      checkpoint_every=4)
+    psib = api.Batch(8, s.np_part, True, ff=gpu.from_numpy(psi0.copy()).cuda())
+    phib = api.Batch(8, s.np_part, True, ff=gpu.from_numpy(phi.copy()).cuda())
+    adj.forward(psib, eps)
+    out = []
+    for _ in range(2):
+        J, chib = overlap_objective(phib, psib, s.np, vol)
+        deps, dv = adj.backward(chib)
+        gpu.cuda.synchronize()
+        out.append((np.array(deps), dv.cpu().numpy(), chib.ff.cpu().numpy()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
 
 
 def test_adjoint_matches_central_differences(gpu):
