@@ -103,6 +103,8 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s, doub
                                   (size_t)(np_part - np) * L)
                 : launch_bra(b, h->nchunk, false, s);
   if (rc) return rc;
+  // with the MFMA ket, each ket workgroup sums the partials of its matrix itself
+  const bool fuse = mfma && h->max_nprojs <= KET_FUSE_PJMAX;
   BraReduceArgs r;
   r.partial = h->d_partial;
   r.proj_mat = h->d_proj_mat;
@@ -113,9 +115,15 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s, doub
   r.L = L;
   r.pjmax = h->max_nprojs;
   r.nproj = h->full_projection_size;
-  rc = launch_bra_reduce(r, s);
+  if (!fuse) rc = launch_bra_reduce(r, s);
   if (rc) return rc;
   KetArgs k;
+  if (fuse) {
+    k.partial = h->d_partial;
+    k.mat_chunk0 = h->d_mat_chunk0;
+    k.scals = h->d_scals;
+    k.pjmax = h->max_nprojs;
+  }
   k.chunks = h->d_chunks;
   k.mat_info = h->d_mat_info;
   k.mat_pbase = h->d_mat_pbase;

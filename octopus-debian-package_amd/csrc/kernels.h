@@ -289,7 +289,17 @@ struct KetArgs {
   const int* __restrict__ multi_pos;  // nmulti+1 CSR into multi_g
   const int* __restrict__ multi_g;    // global sphere points, matrix order
   int nmulti, total_points;
+  // optional (MFMA form only): the bra's chunk partials instead of proj; each workgroup
+  // then forms the projections of its matrix and lane tile itself, summed and scaled as
+  // k_bra_reduce does (so bit-identical), and k_bra_reduce is not launched
+  const double* __restrict__ partial = nullptr;
+  const int* __restrict__ mat_chunk0 = nullptr;  // nmat+1
+  const double* __restrict__ scals = nullptr;
+  int pjmax = 0;
 };
+
+// largest projector count per matrix for which the ket forms its projections itself
+constexpr int KET_FUSE_PJMAX = 32;
 
 struct BraMfmaArgs {
   BraArgs b;

@@ -249,6 +249,25 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
   const int kk = ln >> 4, ii = ln & 15;
   const int col = lt * 16 + ii;
   const int ngrp = (ch.z - ch.y + 15) >> 4;
+  __shared__ double sproj[KET_FUSE_PJMAX * 16];
+  if (k.partial) {
+    // proj[pb + j, lt * 16 + i] for j < npj, i < 16: the bra_reduce_body sum, same order
+    const int c0 = k.mat_chunk0[ch.x], c1 = k.mat_chunk0[ch.x + 1];
+    for (int t = threadIdx.x; t < npj * 16; t += BLOCK) {
+      const int j = t >> 4, l = lt * 16 + (t & 15);
+      double s = 0.0;
+      for (int c = c0; c < c1; c += 32) {
+        double v[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) v[u] = k.partial[((size_t)min(c + u, c1 - 1) * k.pjmax + j) * k.L + l];
+#pragma unroll
+        for (int u = 0; u < 32; ++u)
+          if (c + u < c1) s += v[u];
+      }
+      sproj[t] = s * k.scals[pb + j];
+    }
+    __syncthreads();
+  }
   for (int g = wave; g < ngrp; g += 4) {
     const int pt0 = ch.y + g * 16;
     const int q = pt0 + ii;
@@ -258,7 +277,7 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
       const bool jok = jb + kk < npj;
       const int jj = jok ? jb + kk : npj - 1;
       const double a = k.pt[(size_t)(mapoff + qs) * k.pjs + jj];
-      const double bb = k.proj[(size_t)(pb + jj) * k.L + col];
+      const double bb = k.partial ? sproj[jj * 16 + ii] : k.proj[(size_t)(pb + jj) * k.L + col];
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((q < ch.z && jok) ? a : 0.0, jok ? bb : 0.0, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -315,6 +334,9 @@ __global__ __launch_bounds__(BLOCK) void k_ket_sum(const KetArgs k) { ket_sum_bo
 
 int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
   if (nchunk <= 0) return OCTHIP_SUCCESS;
+  if (k.partial && !(mfma && k.L >= 16 && k.pjmax <= KET_FUSE_PJMAX))
+    return fail(OCTHIP_ERR_INVALID, "k_ket: projections from partials need the MFMA form and <= %d projectors",
+                KET_FUSE_PJMAX);
   {
     ProfScope prof("k_ket", s);
     if (mfma && k.L >= 16)
