@@ -423,5 +423,7 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
   OCTHIP_HIP(hipGetLastError());
   OCTHIP_HIP(hipMemcpyAsync(dj_deps, a->d_deps, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost, s));
   OCTHIP_HIP(hipStreamSynchronize(s));
-  return OCTHIP_SUCCESS;
+  // the Hamiltonian's work set of the recompute stream is not kept between backwards, so
+  // adjoints created and destroyed on one Hamiltonian leave nothing behind in it
+  return octhip_hamiltonian_release_stream(a->h, s2);
 }
