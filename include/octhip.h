@@ -218,7 +218,8 @@ int octhip_batched_destroy(octhip_batched_t batched);
  * backward recomputes each segment from its checkpoint, the next segment on an internal
  * second stream while the current one runs backward, and accumulates the gradients on a
  * third (memory: ceil(nsteps/K) batches of np_part rows plus 2*(K+1) + 2*K*(order-1) +
- * 2*order + 2 batches of np rows).
+ * 2*order + 2 batches of np rows; the Hamiltonian's work set for the second stream exists
+ * only while backward runs).
  * create uploads positions (3*np doubles, host) into the Hamiltonian; forward leaves the
  * Hamiltonian's dipole field at the field of its last step (backward forms its own
  * potentials and leaves the field as it is). */
