@@ -108,9 +108,11 @@ int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const double* fi, 
 
 /* Same contract as the CPU leaf doperate_ri_vec/zoperate_ri_vec
  * (src/grid/operate.c:40-76), on device memory: ri, rimap_inv, rimap_inv_max, fi, fo
- * are device pointers, w is a host array of n doubles.  Runs l = 0..nri-1 cover
- * points [rimap_inv[l], rimap_inv_max[l]).  Stateless, for callers that keep Octopus's
- * own index buffers on the device. */
+ * are device pointers, w is a host array of n doubles (read during the call; it travels
+ * as a kernel argument).  Runs l = 0..nri-1 cover points [rimap_inv[l],
+ * rimap_inv_max[l]).  Stateless, for callers that keep Octopus's own index buffers on
+ * the device: allocates nothing and returns without synchronising (asynchronous on
+ * `stream`, like every other entry point). */
 int octhip_operate_ri_vec(int n, const double* w, int nri, const int* ri,
                           const int* rimap_inv, const int* rimap_inv_max,
                           const double* fi, int ldf, double* fo, void* stream);
@@ -172,9 +174,13 @@ int octhip_exponential_taylor_batch(octhip_hamiltonian_t hm, double* psi, int ld
  * te%apply_batch being octhip_exponential_taylor_batch, without per-kernel launch cost.  The graph holds the batch pointers,
  * deltat, order and the handle's potential buffers as they are at capture;
  * set_potential and set_dipole_field update those buffers in place, so a field that
- * changes between launches is seen when one was set before capture.  The graph shares
- * the handle's work buffers: launches that overlap with other work on the same handle
- * race.  stream == NULL captures on a private stream; the graph may be launched on any
+ * changes between launches is seen when one was set before capture.  The graph owns the
+ * work buffers it was captured with (one set per branch; destroy frees them, after
+ * waiting for the device); only the potential buffers are shared with the handle, so
+ * launches overlapping a set_potential/set_dipole_field race with it.  Destroy the
+ * graphs of a Hamiltonian before or after it: octhip_hamiltonian_destroy frees the sets
+ * of graphs still alive, which then own nothing but may no longer be launched.
+ * stream == NULL captures on a private stream; the graph may be launched on any
  * stream. */
 int octhip_taylor_graph_create(octhip_hamiltonian_t hm, double* const* psi, int nbatch, int ldf,
                                double deltat, int order, int imag_time, void* stream,
@@ -217,7 +223,7 @@ int octhip_batched_destroy(octhip_batched_t batched);
  * leaves chi_0 = dJ/d psi_0* in chi.  Forward stores psi every checkpoint_every steps;
  * backward recomputes each segment from its checkpoint, the next segment on an internal
  * second stream while the current one runs backward, and accumulates the gradients on a
- * third (memory: ceil(nsteps/K) batches of np_part rows plus 2*(K+1) + 2*K*(order-1) +
+ * third (memory: ceil(nsteps/K) checkpoints of np rows plus 2*(K+1) + 2*K*(order-1) +
  * 2*order + 2 batches of np rows; the Hamiltonian's work set for the second stream exists
  * only while backward runs).
  * create uploads positions (3*np doubles, host) into the Hamiltonian; forward leaves the
@@ -256,7 +262,7 @@ int octhip_profiling_select(const char* region);
  *   XCD_CHUNK        gather: consecutive point tiles given to one XCD (1 = round-robin)
  *   GATHER_TILE      gather: passes over a workgroup's point slots per tile (1..8)
  *   ZSEG             z-march: output planes per work item (1..64)
- *   LANE_CHUNK       z-march: lanes of a row handled by one workgroup (1..16, power of 2;
+ *   LANE_CHUNK       z-march: lanes of a row handled by one workgroup (1..32, power of 2;
  *                    0 = min(row length, 16))
  *   LANES_PER_THREAD z-march: 1 or 2 doubles per thread (0 = automatic)
  *   BRA_MODE         projector bra: 0 plain FMA reduction, 1 FP64 MFMA

@@ -288,8 +288,10 @@ extern "C" int octhip_adjoint_forward(octhip_adjoint_t a, double* psi, const dou
   return OCTHIP_SUCCESS;
 }
 
-extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* dj_deps, double* dj_dv,
-                                       void* stream) {
+namespace {
+// The backward pass proper; octhip_adjoint_backward releases the recompute stream's work
+// set after it whether it succeeded or not.
+int backward_run(octhip_adjoint_s* a, double* chi, double* dj_deps, double* dj_dv, void* stream) {
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(a && chi && dj_deps && dj_dv, "null argument");
   OCTHIP_REQUIRE(a->forward_done, "adjoint_backward before adjoint_forward");
@@ -423,7 +425,23 @@ extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* 
   OCTHIP_HIP(hipGetLastError());
   OCTHIP_HIP(hipMemcpyAsync(dj_deps, a->d_deps, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost, s));
   OCTHIP_HIP(hipStreamSynchronize(s));
-  // the Hamiltonian's work set of the recompute stream is not kept between backwards, so
-  // adjoints created and destroyed on one Hamiltonian leave nothing behind in it
-  return octhip_hamiltonian_release_stream(a->h, s2);
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* dj_deps, double* dj_dv,
+                                       void* stream) {
+  const int rc = backward_run(a, chi, dj_deps, dj_dv, stream);
+  if (!a) return rc;
+  // The Hamiltonian's work set of the recompute stream is not kept between backwards, so
+  // adjoints created and destroyed on one Hamiltonian leave nothing behind in it.  After
+  // a failure the work already queued on the three streams drains first.
+  if (rc) {
+    (void)hipStreamSynchronize(a->aux);
+    (void)hipStreamSynchronize(a->accs);
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    (void)octhip_hamiltonian_release_stream(a->h, a->aux);
+    return rc;
+  }
+  return octhip_hamiltonian_release_stream(a->h, a->aux);
 }

@@ -1,4 +1,5 @@
 // Hamiltonian handle: local potential, KB projectors, fused apply, Taylor propagator.
+#include <algorithm>
 #include <complex>
 #include <functional>
 #include <cstring>
@@ -18,6 +19,9 @@ int prepare_operator(octhip_nl_operator_s* op, int L, bool nl);
 namespace {
 
 enum { O_POINTS = 0, O_PROJS, O_MATRIX, O_MAP, O_SCAL, O_MIX };
+
+// a graph whose Hamiltonian is destroyed no longer owns work sets (they are freed with it)
+void detach_graph(octhip_taylor_graph_t g);
 
 // the work-buffer set of stream s, created empty on first use, and mirrored into h
 octhip_hamiltonian_s::WorkSet& workset(octhip_hamiltonian_s* h, hipStream_t s) {
@@ -76,8 +80,6 @@ int ensure_taylor(octhip_hamiltonian_s* h, int ldf, hipStream_t s) {
   bind(h, w);
   return OCTHIP_SUCCESS;
 }
-
-int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s);
 
 // projections of batch f (L lanes) into h->d_proj; zero_bc (optional) = a batch whose
 // boundary rows are cleared on the way (fused into the bra launch when it can be)
@@ -150,6 +152,18 @@ __global__ void k_veff(double* __restrict__ veff, const double* __restrict__ v,
 }  // namespace
 
 namespace octhip {
+int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s) {
+  const size_t count = (size_t)(np_part - np) * L;
+  if (count == 0) return OCTHIP_SUCCESS;
+  size_t grid = (count + BLOCK - 1) / BLOCK;
+  if (grid > 8192) grid = 8192;
+  ProfScope prof("k_zero_rows", s);
+  hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)grid), dim3(BLOCK), 0, s, f, (size_t)np * L, count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zero_rows: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+
 // out = v + field . x over the np points (a caller's buffer, e.g. one per stream)
 int launch_veff(const octhip_hamiltonian_s* h, double* out, const double field[3], hipStream_t s) {
   if (!h->d_v || !h->d_xyz) return fail(OCTHIP_ERR_STATE, "launch_veff: no local potential or positions");
@@ -175,18 +189,6 @@ int update_veff(octhip_hamiltonian_s* h, hipStream_t s) {
                      h->field[1], h->field[2], h->np);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_veff: %s", hipGetErrorString(e));
-  return OCTHIP_SUCCESS;
-}
-
-int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s) {
-  const size_t count = (size_t)(np_part - np) * L;
-  if (count == 0) return OCTHIP_SUCCESS;
-  size_t grid = (count + BLOCK - 1) / BLOCK;
-  if (grid > 8192) grid = 8192;
-  ProfScope prof("k_zero_rows", s);
-  hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)grid), dim3(BLOCK), 0, s, f, (size_t)np * L, count);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_zero_rows: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
 
@@ -347,6 +349,7 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   for (auto& w : h->sets)
     for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
   for (void* p : h->retired) dfree(p);
+  for (octhip_taylor_graph_t g : h->graphs) detach_graph(g);
   delete h;
   return OCTHIP_SUCCESS;
 }
@@ -399,12 +402,7 @@ extern "C" int octhip_hamiltonian_release_stream(octhip_hamiltonian_t h, void* s
   for (size_t i = 0; i < h->sets.size(); ++i) {
     if (h->sets[i].s != s) continue;
     OCTHIP_HIP(hipStreamSynchronize(s));
-    const auto w = h->sets[i];
-    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
-    h->sets.erase(h->sets.begin() + (long)i);
-    // the mirrored pointers may name the freed set: they are rebound by the next call
-    h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = h->d_y[2] = nullptr;
-    h->work_ldf = h->y_ldf = -1;
+    drop_workset(h, s);
     break;
   }
   return OCTHIP_SUCCESS;
@@ -499,7 +497,17 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
 struct octhip_taylor_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the Hamiltonian whose work sets this graph was captured with, and their keys there
+  octhip_hamiltonian_s* h = nullptr;
+  std::vector<hipStream_t> set_keys;
 };
+
+namespace {
+void detach_graph(octhip_taylor_graph_t g) {
+  g->h = nullptr;
+  g->set_keys.clear();
+}
+}  // namespace
 
 extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf,
                                           double deltat, int order, int imag_time, void* stream,
@@ -555,11 +563,25 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
                          return r;
                        },
                        out);
-  // the graph owns the buffers it was captured with: later calls on any stream get others
-  pin_workset(h, used);
-  for (int k = 1; k < nbr; ++k) pin_workset(h, br[k]);
+  if (rc) {
+    // the sets prepared for streams that no longer exist (the branch streams, and the
+    // private capturing stream when stream == NULL) go; the caller's stream keeps its own
+    for (int k = 1; k < nbr; ++k) drop_workset(h, br[k]);
+    if (!stream && used) drop_workset(h, used);
+    cleanup();
+    return rc;
+  }
+  // the graph owns the buffers it was captured with (later calls on any stream get
+  // others) and frees them when it is destroyed
+  octhip_taylor_graph_t g = *out;
+  g->h = h;
+  for (int k = 0; k < nbr; ++k) {
+    const hipStream_t key = pin_workset(h, k == 0 ? used : br[k]);
+    if (key) g->set_keys.push_back(key);
+  }
+  h->graphs.push_back(g);
   cleanup();
-  return rc;
+  return OCTHIP_SUCCESS;
 }
 
 namespace octhip {
@@ -608,9 +630,26 @@ int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
   return OCTHIP_SUCCESS;
 }
 
-void pin_workset(octhip_hamiltonian_s* h, hipStream_t s) {
+hipStream_t pin_workset(octhip_hamiltonian_s* h, hipStream_t s) {
   for (auto& w : h->sets)
-    if (w.s == s) w.s = reinterpret_cast<hipStream_t>(static_cast<intptr_t>(-(++h->graph_sets)));
+    if (w.s == s) {
+      w.s = reinterpret_cast<hipStream_t>(static_cast<intptr_t>(-(++h->graph_sets)));
+      return w.s;
+    }
+  return nullptr;
+}
+
+void drop_workset(octhip_hamiltonian_s* h, hipStream_t s) {
+  for (size_t i = 0; i < h->sets.size(); ++i) {
+    if (h->sets[i].s != s) continue;
+    const auto w = h->sets[i];
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
+    h->sets.erase(h->sets.begin() + (long)i);
+    // the mirrored pointers may name the freed set: they are rebound by the next call
+    h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = h->d_y[2] = nullptr;
+    h->work_ldf = h->y_ldf = -1;
+    return;
+  }
 }
 }  // namespace octhip
 
@@ -624,6 +663,13 @@ extern "C" int octhip_taylor_graph_launch(octhip_taylor_graph_t g, void* stream)
 
 extern "C" int octhip_taylor_graph_destroy(octhip_taylor_graph_t g) {
   if (!g) return OCTHIP_SUCCESS;
+  if (g->h) {
+    // launches of the graph may still be running on any stream
+    (void)hipDeviceSynchronize();
+    for (hipStream_t key : g->set_keys) drop_workset(g->h, key);
+    auto& gs = g->h->graphs;
+    gs.erase(std::remove(gs.begin(), gs.end(), g), gs.end());
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;

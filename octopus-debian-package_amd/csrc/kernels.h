@@ -246,6 +246,8 @@ static __global__ void k_zaxpy(double* __restrict__ y, const double* __restrict_
 
 // launch helper: picks the template instance for (n, index kind, lanes, epilogue)
 int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s);
+// zero rows np..np_part-1 of a batch of L lanes (boundaries_set, zero Dirichlet)
+int zero_boundary(double* f, int np, int np_part, int L, hipStream_t s);
 
 // ---------------------------------------------------------------- projectors --
 struct BraArgs {

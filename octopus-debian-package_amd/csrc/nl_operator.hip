@@ -596,20 +596,27 @@ extern "C" int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const d
 
 // ------------------------------------------------------- stateless leaf --
 namespace {
-// one wave per run; the wave walks the run's points, lanes walk the row
-__global__ __launch_bounds__(BLOCK) void k_operate_ri(int n, const double* __restrict__ w, int nri,
-                                                      const int* __restrict__ ri,
-                                                      const int* __restrict__ imin,
-                                                      const int* __restrict__ imax,
-                                                      const double* __restrict__ fi, int L,
+struct RiWeights {
+  double w[MAX_STENCIL];
+};
+
+// tpr = min(L, 64) consecutive threads own one run and walk its points, each thread a
+// lane (stride tpr) of every point; a 256-thread workgroup covers 256 / tpr runs, so
+// short runs (two points on average in Octopus's blocked order) still fill the waves.
+// Per lane the sum runs over j in stencil order with fused multiply-adds, as the C leaf.
+__global__ __launch_bounds__(BLOCK) void k_operate_ri(int n, const RiWeights w, int nri, const int* __restrict__ ri,
+                                                      const int* __restrict__ imin, const int* __restrict__ imax,
+                                                      const double* __restrict__ fi, int L, int tshift,
                                                       double* __restrict__ fo) {
-  const int l = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)));
+  const int tpr = 1 << tshift;
+  const int l = (int)(blockIdx.x * (BLOCK >> tshift)) + (int)(threadIdx.x >> tshift);
   if (l >= nri) return;
   const int* row = ri + (size_t)l * n;
-  for (int i = imin[l]; i < imax[l]; ++i)
-    for (int k = threadIdx.x & 63; k < L; k += 64) {
+  const int i0 = imin[l], i1 = imax[l];
+  for (int i = i0; i < i1; ++i)
+    for (int k = threadIdx.x & (tpr - 1); k < L; k += tpr) {
       double a = 0.0;
-      for (int j = 0; j < n; ++j) a = fma(w[j], fi[(size_t)(row[j] + i) * L + k], a);
+      for (int j = 0; j < n; ++j) a = fma(w.w[j], fi[(size_t)(row[j] + i) * L + k], a);
       fo[(size_t)i * L + k] = a;
     }
 }
@@ -623,17 +630,14 @@ extern "C" int octhip_operate_ri_vec(int n, const double* w, int nri, const int*
   OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);
   if (nri == 0) return OCTHIP_SUCCESS;
   OCTHIP_REQUIRE(w && ri && rimap_inv && rimap_inv_max && fi && fo, "null argument");
-  hipStream_t s = (hipStream_t)stream;
-  double* dw = nullptr;
-  int rc = dalloc((void**)&dw, sizeof(double) * n);
-  if (rc) return rc;
-  OCTHIP_HIP(hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  const int waves = BLOCK / 64;
-  hipLaunchKernelGGL(k_operate_ri, dim3((nri + waves - 1) / waves), dim3(BLOCK), 0, s, n, dw, nri, ri,
-                     rimap_inv, rimap_inv_max, fi, 1 << ldf, fo);
+  // the weights travel as a kernel argument: nothing is allocated, nothing synchronised
+  RiWeights rw;
+  for (int j = 0; j < MAX_STENCIL; ++j) rw.w[j] = j < n ? w[j] : 0.0;
+  const int tshift = ldf < 6 ? ldf : 6;
+  const int runs_per_block = BLOCK >> tshift;
+  hipLaunchKernelGGL(k_operate_ri, dim3((nri + runs_per_block - 1) / runs_per_block), dim3(BLOCK), 0,
+                     (hipStream_t)stream, n, rw, nri, ri, rimap_inv, rimap_inv_max, fi, 1 << ldf, tshift, fo);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  dfree(dw);
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "operate_ri_vec: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }

@@ -182,6 +182,9 @@ struct octhip_hamiltonian_s {
   };
   std::vector<WorkSet> sets;
   long graph_sets = 0;
+  // Taylor graphs holding sets of this handle (each frees its own sets when destroyed;
+  // destroying the handle first frees them all and detaches the graphs)
+  std::vector<octhip_taylor_graph_t> graphs;
   int work_ldf = -1;
   double* d_partial = nullptr;  // nchunk * max_nprojs * L
   double* d_proj = nullptr;     // full_projection_size * L
@@ -210,8 +213,11 @@ void taylor_coefs(double deltat, int imag_time, int order, std::complex<double>*
 // prepare(capturing stream) runs before the capture starts (allocations go there)
 int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
                   const std::function<int(hipStream_t)>& body, octhip_taylor_graph_t* out);
-// the work-buffer set of stream s is taken out of the per-stream lookup (after a capture)
-void pin_workset(octhip_hamiltonian_s* h, hipStream_t s);
+// the work-buffer set of stream s is taken out of the per-stream lookup (after a capture);
+// returns the key it is kept under from then on (0 if stream s had no set)
+hipStream_t pin_workset(octhip_hamiltonian_s* h, hipStream_t s);
+// free the work-buffer set kept under key s (no synchronisation: nothing may use it)
+void drop_workset(octhip_hamiltonian_s* h, hipStream_t s);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
 int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
 }  // namespace octhip

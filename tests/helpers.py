@@ -43,3 +43,27 @@ def random_packed(np_part: int, lanes: int, seed: int, np_inner: int | None = No
     if np_inner is not None:
         a[np_inner:] = 0.0
     return a
+
+
+def synthetic_projectors(mesh, centers, rc: float, nprojs, seed: int):
+    """Flat build_accel arrays (ProjectorSet) with arbitrary projector counts per matrix:
+    matrix a covers the inner points within rc of centers[a] and holds nprojs[a] seeded
+    random columns; scal ~ U[-1, 1] * vol_pp.  Spheres may overlap."""
+    from octhip.projectors import ProjectorSet, sphere_points
+    r = np.random.default_rng(seed)
+    offs, mats, maps, scals = [], [], [], []
+    moff = poff = soff = 0
+    for c, npj in zip(centers, nprojs):
+        idx = sphere_points(mesh, np.asarray(c, np.float64), rc)
+        npt = len(idx)
+        P = r.standard_normal((npt, npj))
+        offs.append([npt, npj, moff, poff, soff, -1])
+        mats.append(P.T.ravel())
+        maps.append((idx + 1).astype(np.int32))
+        scals.append(r.uniform(-1.0, 1.0, npj) * mesh.vol_pp)
+        moff += npt * npj
+        poff += npt
+        soff += npj
+    return ProjectorSet(nmat=len(nprojs), offsets=np.asarray(offs, dtype=np.int32).reshape(-1, 6),
+                        matrices=np.concatenate(mats), maps=np.concatenate(maps),
+                        scals=np.concatenate(scals), centers=np.asarray(centers, np.float64).reshape(-1, 3))

@@ -116,3 +116,43 @@ def test_hamiltonian_dipole_field(gpu, bra):
     ref = O.Hamiltonian(s.op, s.v, s.proj, xyz=xyz, field=f).apply(workloads.pack(z, s.np_part),
                                                                     terms=O.TERM_LOCAL)
     assert np.array_equal(hpsib.ff.cpu().numpy(), ref)
+
+
+def test_taylor_graph_create_destroy_frees_work_sets(gpu):
+    """Each Taylor graph owns the work sets it was captured with and frees them when it is
+    destroyed: rebuilding graphs (a new dt, new batches) does not grow device memory."""
+    import gc
+    print("[taylor graph create/destroy memory]", flush=True)
+    s = small_sphere(radius=5.0, spacing=0.25, natoms=3, rc=1.2)
+    hm = api.HamiltonianDevice.from_system(s)
+    te = api.Exponential(4)
+    bs = [api.Batch.from_states(workloads.gaussian_envelope_states(s, 16, width=2.0, first=16 * i), s.np_part)
+          for i in range(4)]
+    # first cycle builds the operator's z-march plans, which live as long as the operator
+    g = api.TaylorGraph(te, hm, bs, 0.01)
+    g.launch()
+    del g
+    gc.collect()
+    gpu.cuda.synchronize()
+    free0, _ = gpu.cuda.mem_get_info()
+    for dt in (0.011, 0.012, 0.013, 0.014):
+        g = api.TaylorGraph(te, hm, bs, dt)
+        g.launch()
+        gpu.cuda.synchronize()
+        del g
+        gc.collect()
+    gpu.cuda.synchronize()
+    free1, _ = gpu.cuda.mem_get_info()
+    # four branches, each with three Taylor buffers of np rows of 32 doubles: far above 16 MB
+    set_bytes = 4 * 3 * s.np * 32 * 8
+    assert set_bytes > 16 << 20
+    assert free0 - free1 < 16 << 20, f"device memory grew by {(free0 - free1) / 2**20:.1f} MB"
+    # a graph outliving its Hamiltonian: destroying the Hamiltonian frees the sets, the
+    # graph then destroys cleanly
+    g = api.TaylorGraph(te, hm, bs, 0.01)
+    hm_handle = hm.handle
+    hm.handle = None
+    from octhip import _lib
+    _lib.call("octhip_hamiltonian_destroy", hm_handle)
+    del g
+    gc.collect()

@@ -205,3 +205,33 @@ def test_taylor_run_drift(gpu):
     assert np.max(err) <= 1e-10
     ng, nr = np.sum(np.abs(got) ** 2, axis=1), np.sum(np.abs(ref) ** 2, axis=1)
     assert np.max(np.abs(ng - nr) / nr) <= 1e-10
+
+
+@pytest.mark.parametrize("ldf", [0, 1, 2, 3, 4, 5, 6])
+def test_operate_ri_vec_stateless_bit_exact(gpu, ldf):
+    """octhip_operate_ri_vec, the device counterpart of doperate_ri_vec, on Octopus's own
+    run-compressed index (ri, rimap_inv, rimap_inv_max as device buffers): bit-identical
+    to the oracle leaf, runs split in two calls as two OpenMP threads would split them."""
+    import ctypes as C
+    from octhip import _lib
+    print(f"[operate_ri_vec ldf={ldf}]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.25)
+    op = s.op
+    L = 1 << ldf
+    f = random_packed(s.np_part, L, seed=100 + ldf)
+    w = np.ascontiguousarray(op.w * -0.5, np.float64)
+    ri = to_dev(gpu, np.ascontiguousarray(op.ri, np.int32).ravel())
+    rinv = np.ascontiguousarray(op.rimap_inv, np.int32)
+    imin, imax = to_dev(gpu, rinv[:-1].copy()), to_dev(gpu, rinv[1:].copy())
+    fi = to_dev(gpu, f)
+    fo = gpu.full((s.np, L), np.nan, dtype=gpu.float64, device="cuda")
+    half = op.nri // 2
+    stream = C.c_void_p(gpu.cuda.current_stream().cuda_stream)
+    for l0, l1 in ((0, half), (half, op.nri)):
+        _lib.call("octhip_operate_ri_vec", op.n, _lib.as_ptr(w, C.c_double), l1 - l0,
+                  C.c_void_p(ri.data_ptr() + 4 * op.n * l0), C.c_void_p(imin.data_ptr() + 4 * l0),
+                  C.c_void_p(imax.data_ptr() + 4 * l0), C.c_void_p(fi.data_ptr()), ldf,
+                  C.c_void_p(fo.data_ptr()), stream)
+    gpu.cuda.synchronize()
+    ref = O.operate_ri_vec(op, f, -0.5)
+    assert np.array_equal(fo.cpu().numpy(), ref)
