@@ -9,7 +9,7 @@ order-4 star stencil, 64 states in two packed batches of 32, 16 Gaussian wells,
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hamiltonian|laplacian|taylor|adjoint|batched]
 
 Prints one JSON line.  `value` is state-points/s with the states already resident in
-HBM; `roofline` is the dominant kernel (k_apply, the fused stencil + local + ket pass)
+HBM; `roofline` is the dominant kernel (k_zmarch, the fused stencil + local + ket pass)
 timed with HIP events on its launch stream over the timed region; `cpu_baseline` is the
 CPU oracle (C/OpenMP restatement of the reference CPU path) on a bounded sample.
 """
@@ -130,7 +130,9 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     torch.cuda.synchronize()
     # timed region: HIP events around the dominant kernel only (every event pair costs
     # device time between kernels)
-    _lib.profiling_select("k_apply")
+    # the fused pass is k_zmarch on the lattice, k_apply (tiled gather) with --no-lattice
+    kname = "k_apply" if args.no_lattice or args.kernel == 1 else "k_zmarch"
+    _lib.profiling_select(kname)
     _lib.profiling_start()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -139,7 +141,9 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     t1 = time.perf_counter()
     _lib.profiling_stop()
     ms_step = (t1 - t0) * 1e3 / args.steps
-    tot, n = _lib.profiling_get("k_apply")
+    tot, n = _lib.profiling_get(kname)
+    if n == 0:
+        raise RuntimeError(f"no {kname} launches were recorded")
     k_apply_us = tot * 1e3 / n
     # per-kernel breakdown from a separate pass with every kernel timed
     _lib.profiling_select(None)
@@ -149,10 +153,10 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
     torch.cuda.synchronize()
     _lib.profiling_stop()
     kern = {}
-    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_zero_rows"):
+    for k in (kname, "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_zero_rows"):
         tot, n = _lib.profiling_get(k)
         kern[k] = {"avg_us": tot * 1e3 / n if n else None, "launches": n}
-    kern["k_apply"]["avg_us_timed_region"] = k_apply_us
+    kern[kname]["avg_us_timed_region"] = k_apply_us
     bytes_batch = hamiltonian_bytes(s, pack)
     t_apply = k_apply_us * 1e-6
     achieved = bytes_batch / t_apply / 1e9
@@ -172,7 +176,7 @@ def bench_hamiltonian(args, torch, api, _lib, workloads):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": args.traffic if args.traffic is not None else profiled_traffic(),
                      "traffic_source": "--traffic" if args.traffic is not None else str(PROFILE.relative_to(ROOT)),
-                     "kernel": "k_apply", "algorithmic_bytes_per_launch": bytes_batch,
+                     "kernel": kname, "algorithmic_bytes_per_launch": bytes_batch,
                      "avg_launch_us": k_apply_us},
         "h_apply_hbm_frac": happly_gbs / HBM_PEAK_GBS,
         "kernels": kern,
@@ -204,7 +208,8 @@ def bench_laplacian(args, torch, api, _lib, workloads):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.profiling_stop()
-    tot, n = _lib.profiling_get("k_apply")
+    kname = "k_apply" if args.kernel == 1 else "k_zmarch"
+    tot, n = _lib.profiling_get(kname)
     t_k = tot * 1e-3 / n
     b = 8 * 8 * (s.np_part + s.np) + 8 * 25
     achieved = b / t_k / 1e9
@@ -214,7 +219,7 @@ def bench_laplacian(args, torch, api, _lib, workloads):
             "data": "synthetic", "config": {"workload": "real Laplacian, 64^3 cube h=0.3, 8 functions, pack 8, "
                                                         "MALL flushed between reps"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_apply",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
                          "algorithmic_bytes_per_launch": b, "avg_launch_us": t_k * 1e6}}
 
 
@@ -252,8 +257,8 @@ def bench_taylor(args, torch, api, _lib, workloads):
         return (t1 - t0) * 1e3 / args.steps
 
     ms_eager = timed(eager, True)
-    kern = {k: _lib.profiling_get(k) for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum",
-                                               "k_zero_rows")}
+    kern = {k: _lib.profiling_get(k) for k in ("k_zmarch", "k_apply", "k_bra", "k_bra_reduce", "k_ket",
+                                               "k_ket_sum", "k_zero_rows")}
     ms_graph = timed(graph.launch, False)
     ms = min(ms_graph, ms_eager)
     return {"metric": "FP64 Taylor-4 step state-point H-applies/s (128 states)",
@@ -327,7 +332,7 @@ def bench_adjoint(args, torch, api, _lib, workloads):
     _lib.profiling_stop()
     acc_ms, acc_n = _lib.profiling_get("k_adj_accum")
     breakdown = {}
-    for k in ("k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_adj_accum", "k_zero_rows"):
+    for k in ("k_zmarch", "k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_adj_accum", "k_zero_rows"):
         t, n = _lib.profiling_get(k)
         breakdown[k] = {"total_ms_per_rep": t, "launches_per_rep": n}
     step_ms = t_b * 1e3 / nsteps

@@ -76,6 +76,21 @@ int octhip_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 int octhip_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int octhip_stream_synchronize(void* stream);
 
+/* -------------------------------------------------------------------- batch --
+ * Device pack/unpack of a batch: the transposes of batch_write_unpacked_to_device and
+ * batch_read_device_to_unpacked (src/grid/batch.F90:625-760) with the kernels
+ * dpack/zpack/dunpack/zunpack (share/opencl/pack.cl:23-67), which back batch_t's
+ * do_pack/do_unpack on the device.  unpacked: nst states, state i starting at element
+ * i*ld (ld >= np elements; an element is one double for real batches, one re/im pair of
+ * doubles for complex ones), as the columns of batch_t%dff_linear / zff_linear.
+ * packed: rows of 2^ldf doubles; state i is element first+i of every row (complex: the
+ * doubles 2(first+i), 2(first+i)+1).  Only rows 0..np-1 and the nst states' elements are
+ * written; other lanes keep their values.  All pointers are device memory. */
+int octhip_batch_pack(const double* unpacked, size_t ld, int nst, int np, int is_complex,
+                      double* packed, int ldf, int first, void* stream);
+int octhip_batch_unpack(const double* packed, int ldf, int first, int nst, int np,
+                        int is_complex, double* unpacked, size_t ld, void* stream);
+
 /* ------------------------------------------------------------- nl_operator --
  * octhip_nl_operator_create replaces the device half of nl_operator_build
  * (src/grid/nl_operator.F90:516-582: buff_ri, buff_map, kernel build).  Inputs are the
@@ -168,6 +183,96 @@ int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t hm, double* psi, double*
 int octhip_exponential_taylor_batch(octhip_hamiltonian_t hm, double* psi, int ldf,
                                     double deltat, int order, int imag_time, void* stream);
 
+/* exponential_apply_batch with its optional arguments (src/electrons/exponential.F90:581-674,
+ * Taylor branch; exponential_taylor_series_batch 677-757):
+ *   psi2 (optional, device, np_part rows, same ldf) = psib2: receives psi's inner rows and
+ *     the series at deltat2 built from the same H^k psi (only its np inner rows are
+ *     written);
+ *   inh (optional, device, np_part rows) = inh_psib: after the homogeneous series a second
+ *     series adds deltat sum_k (-i deltat H)^k/(k+1)! inh (psi2 likewise, after receiving
+ *     the propagated psi again, as the reference's second call does); inh's boundary rows
+ *     are zeroed by its H-apply;
+ *   weights (optional, host, 2^(ldf-1) doubles: kweight*occ per state, 0 for padding) and
+ *     density (device, np doubles): density += sum_states weights |final state|^2 of psi2
+ *     when given, else psi -- the density_calc_accumulate that
+ *     propagation_ops_elec_fuse_density_exp_apply (src/td/propagation_ops_elec.F90:
+ *     261-333) runs after each batch, here fused into the last pass for orders 2..4.
+ * With all optional pointers NULL this is octhip_exponential_taylor_batch. */
+int octhip_exponential_apply_batch(octhip_hamiltonian_t hm, double* psi, int ldf, double deltat,
+                                   int order, int imag_time, double* psi2, double deltat2,
+                                   double* inh, const double* weights, double* density,
+                                   void* stream);
+
+/* density_calc_accumulate on a BATCH_DEVICE_PACKED batch (src/states/density.F90:375-417,
+ * kernels density_real/density_complex of share/opencl/density.cl): density(p) +=
+ * sum_ist weights[ist] |psi(ist, p)|^2 over the np inner rows.  psi, weights
+ * (2^ldf doubles for real rows, 2^(ldf-1) for complex) and density are DEVICE pointers,
+ * as the reference kernel takes buff_weight and buff_density. */
+int octhip_density_accumulate(const double* psi, int ldf, int np, int is_complex,
+                              const double* weights, double* density, void* stream);
+
+/* ------------------------------------------------------------ mesh_batch --
+ * Batched reductions over the np inner rows of packed batches (DEVICE pointers), with the
+ * mesh inner product <a|b> = vol * sum_p conj(a) b (vol = the volume element).
+ * mesh_batch_dotp_vector (src/grid/mesh_batch_inc.F90:379-549): dot = <a_c|b_c> per column
+ * c: complex batches give 2^(ldf-1) (re, im) pairs, real batches 2^ldf doubles (host).
+ * The sums are deterministic (fixed partition and order).  Allocates and frees its
+ * reduction scratch on every call and synchronises the stream. */
+int octhip_mesh_batch_dotp_vector(const double* a, const double* b, int ldf, int np, int is_complex,
+                                  double vol, double* dot, void* stream);
+/* mesh_batch_nrm2 (src/grid/mesh_batch.F90:74-95): nrm2 = sqrt(<a_c|a_c>) per column (host) */
+int octhip_mesh_batch_nrm2(const double* a, int ldf, int np, int is_complex, double vol,
+                           double* nrm2, void* stream);
+/* mesh_batch_orthogonalization, classical Gram-Schmidt (src/grid/mesh_batch_inc.F90:
+ * 1168-1274) of the complex batch phi against the nvec batches psi[k]: all overlaps
+ * <psi_k|phi> first, then phi -= sum_k overlap_k psi_k; then optionally norm (host, nst
+ * doubles) and normalize.  overlap (optional, host): nvec x nst complex pairs,
+ * column-major (the transposed layout the reference returns).  Only the first nst
+ * states of the rows are touched. */
+int octhip_mesh_batch_orthogonalization(int nvec, const double* const* psi, double* phi, int ldf,
+                                        int nst, int np, double vol, int normalize,
+                                        double* overlap, double* norm, void* stream);
+
+/* exponential_lanczos_batch (src/electrons/exponential.F90:766-900): psi <- exp(-i deltat
+ * H) psi (imag_time: deltat -> i deltat) in the Krylov space of up to `order` Lanczos
+ * steps for the first nst states of the complex batch; stops when every state's error
+ * estimate (Saad's Er2) is below lanczos_tol after more than 3 steps, or on a happy
+ * breakdown.  With inh (device, np_part rows, read only): psi += deltat phi(-i deltat H)
+ * inh instead (zlalg_phi).  vol = the volume element of the mesh inner product.
+ * iterations/residual (optional, host): the Krylov dimension used and the largest error
+ * estimate (the reference warns when it exceeds lanczos_tol).  Synchronises the stream
+ * (the projected problems are solved on the host). */
+int octhip_exponential_lanczos_batch(octhip_hamiltonian_t hm, double* psi, int ldf, int nst,
+                                     double deltat, int imag_time, int order, double lanczos_tol,
+                                     const double* inh, double vol, int* iterations,
+                                     double* residual, void* stream);
+/* exponential_cheby_batch (src/electrons/exponential.F90:903-971): Chebyshev expansion of
+ * exp(-i deltat H) of `order` terms with Clenshaw's recurrence, for H's spectrum inside
+ * spectral_middle_point +- spectral_half_span (hamiltonian_elec_span,
+ * src/hamiltonian/hamiltonian_elec.F90:898-909). */
+int octhip_exponential_cheby_batch(octhip_hamiltonian_t hm, double* psi, int ldf, double deltat,
+                                   int order, double spectral_middle_point,
+                                   double spectral_half_span, void* stream);
+
+/* Exponential-midpoint step for independent particles (exponential_midpoint,
+ * src/td/propagator_expmid.F90:52-100): the Hamiltonian is updated to t - dt/2
+ * (field_mid = the dipole field there, 3 doubles; NULL keeps the current potential; the
+ * positions must have been uploaded with octhip_hamiltonian_set_dipole_field), then every
+ * batch psi[b] (device, np_part rows, one ldf) is propagated by dt with Taylor of `order`
+ * and the density of the results accumulated (propagation_ops_elec_fuse_density_exp_apply).
+ * weights: host, nbatch x 2^(ldf-1); density: device, np doubles, zeroed first
+ * (density_calc_init); both may be NULL. */
+int octhip_propagator_expmid(octhip_hamiltonian_t hm, double* const* psi, int nbatch, int ldf,
+                             double dt, int order, const double* field_mid,
+                             const double* weights, double* density, void* stream);
+/* Enforced time-reversal-symmetry step for independent particles (td_etrs,
+ * src/td/propagator_etrs.F90:68-150): every batch propagated by dt/2 with H(t - dt)
+ * (field_old), the Hamiltonian updated to t (field_new), every batch propagated by dt/2
+ * with the density accumulated.  Arguments as octhip_propagator_expmid. */
+int octhip_propagator_etrs(octhip_hamiltonian_t hm, double* const* psi, int nbatch, int ldf,
+                           double dt, int order, const double* field_old, const double* field_new,
+                           const double* weights, double* density, void* stream);
+
 /* One Taylor time step over nbatch batches (psi[b], device, same ldf) captured once into
  * a HIP graph and replayed with a single launch: the loop over batches of
  * propagation_ops_elec_exp_apply (src/td/propagation_ops_elec.F90:232-249), each
@@ -243,7 +348,7 @@ int octhip_adjoint_destroy(octhip_adjoint_t adj);
 /* --------------------------------------------------------------- profiling --
  * Per-kernel device timing with HIP events recorded on the launch stream, the device-side
  * counterpart of profiling_in/profiling_out (src/basic/profiling.F90:503-625).  Regions
- * are kernel names ("k_apply", "k_bra", "k_bra_reduce", "k_zero_rows", "k_zaxpy").
+ * are kernel names ("k_zmarch", "k_apply", "k_bra", "k_bra_reduce", "k_ket", "k_ket_sum", "k_zero_rows", "k_zaxpy", "k_density").
  * start clears earlier records; get synchronises the recorded events and returns the
  * summed milliseconds and the number of launches of one region. */
 int octhip_profiling_start(void);

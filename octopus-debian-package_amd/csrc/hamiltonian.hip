@@ -27,7 +27,7 @@ void detach_graph(octhip_taylor_graph_t g);
 octhip_hamiltonian_s::WorkSet& workset(octhip_hamiltonian_s* h, hipStream_t s) {
   for (auto& w : h->sets)
     if (w.s == s) return w;
-  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr});
+  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0});
   return h->sets.back();
 }
 
@@ -347,8 +347,11 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_chunks);
   dfree(h->d_mat_chunk0);
   for (auto& w : h->sets)
-    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2,
+                    (void*)w.dpart, (void*)w.dw})
+      dfree(p);
   for (void* p : h->retired) dfree(p);
+  drop_krylov(h, nullptr, true);
   for (octhip_taylor_graph_t g : h->graphs) detach_graph(g);
   delete h;
   return OCTHIP_SUCCESS;
@@ -399,12 +402,9 @@ extern "C" int octhip_hamiltonian_release_stream(octhip_hamiltonian_t h, void* s
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
   const hipStream_t s = (hipStream_t)stream;
-  for (size_t i = 0; i < h->sets.size(); ++i) {
-    if (h->sets[i].s != s) continue;
-    OCTHIP_HIP(hipStreamSynchronize(s));
-    drop_workset(h, s);
-    break;
-  }
+  OCTHIP_HIP(hipStreamSynchronize(s));
+  drop_workset(h, s);
+  drop_krylov(h, s, false);
   return OCTHIP_SUCCESS;
 }
 
@@ -643,7 +643,9 @@ void drop_workset(octhip_hamiltonian_s* h, hipStream_t s) {
   for (size_t i = 0; i < h->sets.size(); ++i) {
     if (h->sets[i].s != s) continue;
     const auto w = h->sets[i];
-    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2}) dfree(p);
+    for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2,
+                    (void*)w.dpart, (void*)w.dw})
+      dfree(p);
     h->sets.erase(h->sets.begin() + (long)i);
     // the mirrored pointers may name the freed set: they are rebound by the next call
     h->d_partial = h->d_proj = h->d_ksum = h->d_y[0] = h->d_y[1] = h->d_y[2] = nullptr;
@@ -788,3 +790,293 @@ int taylor_step(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, in
   return OCTHIP_SUCCESS;
 }
 }  // namespace octhip
+
+namespace octhip {
+namespace {
+// density buffers of stream s: nchunk x np partials and the weights of 2^(ldf-1) states
+int ensure_density(octhip_hamiltonian_s* h, int nchunk, hipStream_t s, double** dpart, double** dw) {
+  auto& w = workset(h, s);
+  if (!w.dw) {
+    int rc = dalloc((void**)&w.dw, sizeof(double) << (MAX_LDF - 1));
+    if (rc) return rc;
+  }
+  if (nchunk > 0 && w.dchunks < nchunk) {
+    if (w.dpart) h->retired.push_back(w.dpart);
+    w.dpart = nullptr;
+    w.dchunks = 0;
+    int rc = dalloc((void**)&w.dpart, sizeof(double) * (size_t)nchunk * h->np);
+    if (rc) return rc;
+    w.dchunks = nchunk;
+  }
+  *dpart = w.dpart;
+  *dw = w.dw;
+  return OCTHIP_SUCCESS;
+}
+
+// zfact_k of exponential_taylor_series_batch (src/electrons/exponential.F90:716-720) for a
+// complex deltat: zfact_0 (1, or deltat with an inhomogeneous term) and
+// zfact_k = zfact_{k-1} (-i deltat) / (k [+1]), with the products written out as the
+// oracle does.
+void series_coefs(std::complex<double> dt, bool inh, int order, std::complex<double>* z) {
+  const double mr = -0.0, mi = -1.0;
+  const double sr = mr * dt.real() - mi * dt.imag(), si = mr * dt.imag() + mi * dt.real();
+  z[0] = inh ? dt : std::complex<double>(1.0, 0.0);
+  for (int k = 1; k <= order; ++k) {
+    const int denom = inh ? k + 1 : k;
+    const double tr = z[k - 1].real() * sr - z[k - 1].imag() * si;
+    const double ti = z[k - 1].real() * si + z[k - 1].imag() * sr;
+    z[k] = std::complex<double>(tr / denom, ti / denom);
+  }
+}
+
+// One series of exponential_taylor_series_batch, kernel by kernel: psi2 = psi (np rows)
+// first when given; with inh, psi (psi2) += real(zfact_0) inh and the chain of H-applies
+// starts from inh; each order stores y_k = H y_{k-1} and adds zfact_k y_k to psi and
+// zfact2_k y_k to psi2 (the reference's axpy per order, 733-744).
+int series_generic(octhip_hamiltonian_s* h, double* psi, int ldf, std::complex<double> dt, int order,
+                   double* psi2, std::complex<double> dt2, double* inh, hipStream_t s) {
+  const int L = 1 << ldf;
+  const size_t count = (size_t)h->np * L;
+  int rc = prepare_taylor(h, ldf, s);
+  if (rc) return rc;
+  std::complex<double> z[34], z2[34];
+  series_coefs(dt, inh != nullptr, order, z);
+  series_coefs(dt2, inh != nullptr, order, z2);
+  if (psi2) OCTHIP_HIP(hipMemcpyAsync(psi2, psi, sizeof(double) * count, hipMemcpyDeviceToDevice, s));
+  if (inh) {
+    rc = launch_daxpy(psi, inh, count, z[0].real(), s);
+    if (!rc && psi2) rc = launch_daxpy(psi2, inh, count, z2[0].real(), s);
+    if (rc) return rc;
+  }
+  double* src = inh ? inh : psi;
+  for (int k = 1; k <= order; ++k) {
+    const double* yin = (k == 1) ? src : h->d_y[(k - 2) & 1];
+    double* yout = h->d_y[(k - 1) & 1];
+    // boundaries_set of the chain's input (its H-apply, set_bc default)
+    rc = run_bra(h, yin, L, s, k == 1 ? src : nullptr);
+    if (rc) return rc;
+    if (k == 1 && h->nmat == 0) {
+      rc = zero_boundary(src, h->np, h->kin->np_part, L, s);
+      if (rc) return rc;
+    }
+    ApplyArgs a;
+    std::memset(&a, 0, sizeof(a));
+    fill_terms(h, OCTHIP_TERM_ALL, a);
+    a.fi = yin;
+    a.fo = yout;
+    a.L = L;
+    a.lim = h->np;
+    rc = launch_operator(h->kin, a, EPI_STORE, s);
+    if (!rc) rc = launch_zaxpy(psi, yout, count, z[k].real(), z[k].imag(), s);
+    if (!rc && psi2) rc = launch_zaxpy(psi2, yout, count, z2[k].real(), z2[k].imag(), s);
+    if (rc) return rc;
+  }
+  return OCTHIP_SUCCESS;
+}
+
+// The homogeneous series at orders 2..4 with the deferred accumulation: plain H-apply
+// passes store y_k = H^k psi (k < order); the last pass forms psi (and psi2 with the
+// deltat2 factors, from the same inputs) and, when dpart is given, the density partials of
+// the final states.
+int series_fused(octhip_hamiltonian_s* h, double* psi, int ldf, std::complex<double> dt, int order,
+                 double* psi2, std::complex<double> dt2, double* dpart, const double* dw, hipStream_t s) {
+  const int L = 1 << ldf;
+  int rc = prepare_taylor(h, ldf, s);
+  if (rc) return rc;
+  std::complex<double> z[34], z2[34];
+  series_coefs(dt, false, order, z);
+  series_coefs(dt2, false, order, z2);
+  if (h->nmat == 0) {
+    rc = zero_boundary(psi, h->np, h->kin->np_part, L, s);
+    if (rc) return rc;
+  }
+  for (int k = 1; k <= order; ++k) {
+    const double* yin = (k == 1) ? psi : h->d_y[k - 2];
+    rc = run_bra(h, yin, L, s, (k == 1 && h->nmat > 0) ? psi : nullptr);
+    if (rc) return rc;
+    ApplyArgs a;
+    std::memset(&a, 0, sizeof(a));
+    fill_terms(h, OCTHIP_TERM_ALL, a);
+    a.fi = yin;
+    a.L = L;
+    a.lim = h->np;
+    if (k < order) {
+      a.fo = h->d_y[k - 1];
+      rc = launch_operator(h->kin, a, EPI_STORE, s);
+      if (rc) return rc;
+      continue;
+    }
+    a.acc = psi;
+    if (order >= 3) {
+      a.ya = h->d_y[0];
+      a.car = z[1].real();
+      a.cai = z[1].imag();
+      a.dar = z2[1].real();
+      a.dai = z2[1].imag();
+    }
+    if (order >= 4) {
+      a.yb = h->d_y[1];
+      a.cbr = z[2].real();
+      a.cbi = z[2].imag();
+      a.dbr = z2[2].real();
+      a.dbi = z2[2].imag();
+    }
+    a.has_c1 = 1;
+    a.c1r = z[order - 1].real();
+    a.c1i = z[order - 1].imag();
+    a.c2r = z[order].real();
+    a.c2i = z[order].imag();
+    a.d1r = z2[order - 1].real();
+    a.d1i = z2[order - 1].imag();
+    a.d2r = z2[order].real();
+    a.d2i = z2[order].imag();
+    a.acc2 = psi2;
+    a.dpart = dpart;
+    a.dw = dw;
+    return launch_operator(h->kin, a, EPI_TAYLOR_X, s);
+  }
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+int taylor_apply(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                 const TaylorOptions& o, hipStream_t s) {
+  const int L = 1 << ldf;
+  const std::complex<double> dt = imag_time ? std::complex<double>(0.0, deltat) : std::complex<double>(deltat, 0.0);
+  const std::complex<double> dt2 =
+      imag_time ? std::complex<double>(0.0, o.deltat2) : std::complex<double>(o.deltat2, 0.0);
+  double* dpart = nullptr;
+  double* dw = nullptr;
+  int rc = OCTHIP_SUCCESS;
+  const bool fused = taylor_deferred(order);
+  const bool fuse_density = o.density && fused && !o.inh;
+  int nchunk = 0;
+  if (o.density) {
+    // density partials per lane chunk when the last pass forms them, else none
+    ApplyArgs probe;
+    std::memset(&probe, 0, sizeof(probe));
+    fill_terms(h, OCTHIP_TERM_ALL, probe);
+    probe.L = L;
+    nchunk = fuse_density ? apply_lane_chunks(h->kin, probe) : 0;
+    rc = ensure_density(h, nchunk, s, &dpart, &dw);
+    if (rc) return rc;
+    OCTHIP_HIP(hipMemcpyAsync(dw, o.weights, sizeof(double) * (L / 2), hipMemcpyHostToDevice, s));
+  }
+  if (fused) {
+    rc = series_fused(h, psi, ldf, dt, order, o.psi2, dt2, fuse_density ? dpart : nullptr, dw, s);
+  } else {
+    rc = series_generic(h, psi, ldf, dt, order, o.psi2, dt2, nullptr, s);
+  }
+  if (!rc && o.inh) rc = series_generic(h, psi, ldf, dt, order, o.psi2, dt2, o.inh, s);
+  if (rc || !o.density) return rc;
+  if (fuse_density) return launch_density_reduce(o.density, dpart, nchunk, h->np, s);
+  return launch_density(o.psi2 ? o.psi2 : psi, L, h->np, 1, dw, o.density, s);
+}
+}  // namespace octhip
+
+extern "C" int octhip_exponential_apply_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
+                                              int order, int imag_time, double* psi2, double deltat2,
+                                              double* inh, const double* weights, double* density,
+                                              void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && psi, "null argument");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  OCTHIP_REQUIRE(psi2 != psi && inh != psi && (!psi2 || psi2 != inh), "psi, psi2 and inh must be distinct");
+  OCTHIP_REQUIRE((weights == nullptr) == (density == nullptr), "weights and density go together");
+  TaylorOptions o;
+  o.psi2 = psi2;
+  o.deltat2 = deltat2;
+  o.inh = inh;
+  o.weights = weights;
+  o.density = density;
+  return taylor_apply(h, psi, ldf, deltat, order, imag_time, o, (hipStream_t)stream);
+}
+
+extern "C" int octhip_density_accumulate(const double* psi, int ldf, int np, int is_complex, const double* weights,
+                                         double* density, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(psi && weights && density, "null argument");
+  OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);
+  OCTHIP_REQUIRE(np >= 0, "np < 0");
+  return launch_density(psi, 1 << ldf, np, is_complex ? 1 : 0, weights, density, (hipStream_t)stream);
+}
+
+namespace {
+// density = 0 over np doubles with a fill kernel (density_calc_init)
+int clear_density(double* density, int np, hipStream_t s) {
+  const unsigned grid = (unsigned)std::min((np + BLOCK - 1) / BLOCK, 8192);
+  hipLaunchKernelGGL(k_zero_rows, dim3(grid), dim3(BLOCK), 0, s, density, (size_t)0, (size_t)np);
+  OCTHIP_HIP(hipGetLastError());
+  return OCTHIP_SUCCESS;
+}
+
+// propagation_ops_elec_exp_apply / _fuse_density_exp_apply over the batches
+// (src/td/propagation_ops_elec.F90:213-333): every batch propagated by dt in turn, the
+// density of the results accumulated when requested
+int exp_apply_batches(octhip_hamiltonian_s* h, double* const* psi, int nbatch, int ldf, double dt, int order,
+                      const double* weights, double* density, hipStream_t s) {
+  const int pack = 1 << (ldf - 1);
+  if (density) {
+    int rc = clear_density(density, h->np, s);
+    if (rc) return rc;
+  }
+  for (int b = 0; b < nbatch; ++b) {
+    TaylorOptions o;
+    if (density) {
+      o.weights = weights + (size_t)b * pack;
+      o.density = density;
+    }
+    int rc = taylor_apply(h, psi[b], ldf, dt, order, 0, o, s);
+    if (rc) return rc;
+  }
+  return OCTHIP_SUCCESS;
+}
+
+int check_driver_args(octhip_hamiltonian_s* h, double* const* psi, int nbatch, int ldf, int order,
+                      const double* weights, double* density) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && psi && nbatch > 0, "bad argument");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "propagators need complex batches (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  OCTHIP_REQUIRE((weights == nullptr) == (density == nullptr), "weights and density go together");
+  for (int b = 0; b < nbatch; ++b) OCTHIP_REQUIRE(psi[b] != nullptr, "batch %d is null", b);
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+extern "C" int octhip_propagator_expmid(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf, double dt,
+                                        int order, const double* field_mid, const double* weights, double* density,
+                                        void* stream) {
+  int rc = check_driver_args(h, psi, nbatch, ldf, order, weights, density);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (field_mid) {
+    OCTHIP_REQUIRE(h->d_xyz != nullptr, "propagator_expmid: positions never uploaded (set_dipole_field)");
+    rc = set_field(h, field_mid, s);
+    if (rc) return rc;
+  }
+  return exp_apply_batches(h, psi, nbatch, ldf, dt, order, weights, density, s);
+}
+
+extern "C" int octhip_propagator_etrs(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf, double dt,
+                                     int order, const double* field_old, const double* field_new,
+                                     const double* weights, double* density, void* stream) {
+  int rc = check_driver_args(h, psi, nbatch, ldf, order, weights, density);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  OCTHIP_REQUIRE(!(field_old || field_new) || h->d_xyz != nullptr,
+                 "propagator_etrs: positions never uploaded (set_dipole_field)");
+  if (field_old) {
+    rc = set_field(h, field_old, s);
+    if (rc) return rc;
+  }
+  // dt/2 with H(t - dt), then the Hamiltonian at t and dt/2 with the density
+  rc = exp_apply_batches(h, psi, nbatch, ldf, 0.5 * dt, order, nullptr, nullptr, s);
+  if (rc) return rc;
+  if (field_new) {
+    rc = set_field(h, field_new, s);
+    if (rc) return rc;
+  }
+  return exp_apply_batches(h, psi, nbatch, ldf, 0.5 * dt, order, weights, density, s);
+}

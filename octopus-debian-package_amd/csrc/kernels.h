@@ -12,7 +12,10 @@
 namespace octhip {
 
 enum { IDX_LATTICE = 0, IDX_RI = 1 };
-enum { EPI_STORE = 0, EPI_TAYLOR = 1 };
+// EPI_TAYLOR_X: the Taylor epilogue plus an optional second accumulator (psib2 at deltat2)
+// and optional density partials of the final states (separate instances, so the plain
+// Taylor epilogue keeps its register budget)
+enum { EPI_STORE = 0, EPI_TAYLOR = 1, EPI_TAYLOR_X = 2 };
 
 struct ApplyArgs {
   const double* __restrict__ fi;   // input batch (>= lim rows)
@@ -52,6 +55,14 @@ struct ApplyArgs {
   const double* ya;
   const double* yb;
   double car, cai, cbr, cbi;
+  // EPI_TAYLOR_X: acc2 = acc_in + da ya + db yb + d1 y + d2 Hy (same inputs, own factors;
+  // d1 used when has_c1), and density partials dpart[chunk * np + p] = sum over the
+  // chunk's lanes of dw[state] |final value|^2, the final value being acc2 when given,
+  // else acc (dw: one weight per complex state of the row)
+  double* acc2;
+  double d1r, d1i, d2r, d2i, dar, dai, dbr, dbi;
+  double* dpart;
+  const double* dw;
 };
 
 __device__ __forceinline__ int tile_of(int b, int chunk) {
@@ -98,6 +109,12 @@ __device__ __forceinline__ void stv(double* p, const Vec<LPT>& r) {
   } else {
     *p = r.v[0];
   }
+}
+
+// base row piece of the Taylor accumulators: acc_in, or acc itself when in place
+template <int LPT>
+__device__ __forceinline__ Vec<LPT> stv_base(const ApplyArgs& a, int p, int l) {
+  return ldv<LPT>((a.acc_in ? a.acc_in : a.acc) + (size_t)p * a.L + l);
 }
 
 // Fused apply over a tile of `tpts` consecutive points per workgroup.
@@ -159,6 +176,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const ApplyArgs a) {
     const int* qi = sidx + t * ss;
     const double vv = a.v ? a.v[p] : 0.0;
     const int ks = a.kslot ? a.kslot[p] : -1;
+    double dsum = 0.0;  // EPI_TAYLOR_X: this thread's share of the point's density
     for (int l = lane0; l < a.L; l += tpp * LPT) {
       Vec<LPT> s;
 #pragma unroll
@@ -219,8 +237,42 @@ __global__ __launch_bounds__(BLOCK) void k_apply(const ApplyArgs a) {
           }
           av.v[0] = av.v[0] + zmul_lane(a.c2r, a.c2i, s.v[0], s.v[1], false);
           av.v[1] = av.v[1] + zmul_lane(a.c2r, a.c2i, s.v[1], s.v[0], true);
+          if constexpr (EPI == EPI_TAYLOR_X) {
+            if (a.acc2) {
+              // psib2: the same terms with the deltat2 factors, from the same base
+              const Vec<LPT> b0 = stv_base<LPT>(a, p, l);
+              Vec<LPT> a2 = b0;
+              if (a.ya) {
+                const Vec<LPT> u = ldv<LPT>(a.ya + (size_t)p * L + l);
+                a2.v[0] = a2.v[0] + zmul_lane(a.dar, a.dai, u.v[0], u.v[1], false);
+                a2.v[1] = a2.v[1] + zmul_lane(a.dar, a.dai, u.v[1], u.v[0], true);
+              }
+              if (a.yb) {
+                const Vec<LPT> u = ldv<LPT>(a.yb + (size_t)p * L + l);
+                a2.v[0] = a2.v[0] + zmul_lane(a.dbr, a.dbi, u.v[0], u.v[1], false);
+                a2.v[1] = a2.v[1] + zmul_lane(a.dbr, a.dbi, u.v[1], u.v[0], true);
+              }
+              if (a.has_c1) {
+                a2.v[0] = a2.v[0] + zmul_lane(a.d1r, a.d1i, x.v[0], x.v[1], false);
+                a2.v[1] = a2.v[1] + zmul_lane(a.d1r, a.d1i, x.v[1], x.v[0], true);
+              }
+              a2.v[0] = a2.v[0] + zmul_lane(a.d2r, a.d2i, s.v[0], s.v[1], false);
+              a2.v[1] = a2.v[1] + zmul_lane(a.d2r, a.d2i, s.v[1], s.v[0], true);
+              stv<LPT>(a.acc2 + (size_t)p * L + l, a2);
+              if (a.dpart) dsum = fma(a.dw[l >> 1], a2.v[0] * a2.v[0] + a2.v[1] * a2.v[1], dsum);
+            } else if (a.dpart) {
+              dsum = fma(a.dw[l >> 1], av.v[0] * av.v[0] + av.v[1] * av.v[1], dsum);
+            }
+          }
           stv<LPT>(a.acc + (size_t)p * L + l, av);
         }
+      }
+    }
+    if constexpr (EPI == EPI_TAYLOR_X) {
+      // the tpp threads of this point are consecutive lanes of one wave
+      if (a.dpart) {
+        for (int off = tpp >> 1; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off);
+        if ((threadIdx.x & (tpp - 1)) == 0) a.dpart[p] = dsum;
       }
     }
   }
@@ -243,6 +295,17 @@ static __global__ void k_zaxpy(double* __restrict__ y, const double* __restrict_
     y[i] = y[i] + zmul_lane(zr, zi, xv, xq, (i & 1) != 0);
   }
 }
+
+// streaming helpers of the propagators (csrc/propagation.hip)
+int launch_zaxpy(double* y, const double* x, size_t count, double zr, double zi, hipStream_t s);
+int launch_daxpy(double* y, const double* x, size_t count, double a, hipStream_t s);
+// density[p] += sum_states w[state] |psi(state, p)|^2 over the np inner rows
+int launch_density(const double* psi, int L, int np, int is_complex, const double* w, double* density,
+                   hipStream_t s);
+// density[p] += sum_{c < nchunk} dpart[c * np + p], in chunk order
+int launch_density_reduce(double* density, const double* dpart, int nchunk, int np, hipStream_t s);
+// lane chunks of the density partials an EPI_TAYLOR_X pass of this operator writes
+int apply_lane_chunks(const octhip_nl_operator_s* op, const ApplyArgs& a);
 
 // launch helper: picks the template instance for (n, index kind, lanes, epilogue)
 int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s);

@@ -33,6 +33,7 @@ int launch_apply_lpt(ApplyArgs& a, hipStream_t s) {
 template <int IDX>
 int launch_apply_idx(ApplyArgs& a, int epi, int lpt, hipStream_t s) {
   if (epi == EPI_TAYLOR) return launch_apply_lpt<IDX, EPI_TAYLOR, 2>(a, s);
+  if (epi == EPI_TAYLOR_X) return launch_apply_lpt<IDX, EPI_TAYLOR_X, 2>(a, s);
   return lpt == 2 ? launch_apply_lpt<IDX, EPI_STORE, 2>(a, s) : launch_apply_lpt<IDX, EPI_STORE, 1>(a, s);
 }
 
@@ -41,7 +42,7 @@ int launch_apply_idx(ApplyArgs& a, int epi, int lpt, hipStream_t s) {
 int launch_apply(ApplyArgs& a, bool lattice, int epi, hipStream_t s) {
   if (a.np <= 0) return OCTHIP_SUCCESS;
   const int lpt = a.L >= 2 ? 2 : 1;
-  if (epi == EPI_TAYLOR && lpt != 2) return fail(OCTHIP_ERR_INVALID, "Taylor epilogue needs complex rows");
+  if (epi != EPI_STORE && lpt != 2) return fail(OCTHIP_ERR_INVALID, "Taylor epilogue needs complex rows");
   int tpp = a.L / lpt;
   if (tpp > 64) tpp = 64;
   int tshift = 0;

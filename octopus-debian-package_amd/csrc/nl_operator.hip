@@ -370,8 +370,8 @@ int prepare_operator(octhip_nl_operator_s* op, int L, bool nl) {
   if (!zm) return OCTHIP_SUCCESS;
   int lc, lpt;
   zmarch_shape(op, L, &lc, &lpt);
-  for (int epi : {EPI_STORE, EPI_TAYLOR}) {
-    if (epi == EPI_TAYLOR && lc < 2) continue;
+  for (int epi : {EPI_STORE, EPI_TAYLOR, EPI_TAYLOR_X}) {
+    if (epi != EPI_STORE && lc < 2) continue;
     int rc = zmarch_items(op, zmarch_zseg_for(op, L, epi, nl));
     if (rc) return rc;
     const int2* sched = nullptr;
@@ -380,6 +380,16 @@ int prepare_operator(octhip_nl_operator_s* op, int L, bool nl) {
     if (rc) return rc;
   }
   return OCTHIP_SUCCESS;
+}
+
+// The z-march writes one density partial per lane chunk of a row, the gather kernel one
+// per row (keep in step with launch_operator's kernel choice).
+int apply_lane_chunks(const octhip_nl_operator_s* op, const ApplyArgs& a) {
+  const bool zm = op->zmarch_ok && op->kernel != 1 && a.n == 25 && zmarch_fits(op, a);
+  if (!zm) return 1;
+  int lc, lpt;
+  zmarch_shape(op, a.L, &lc, &lpt);
+  return a.L / lc;
 }
 
 // Apply through the best kernel for this operator: z-march for the order-4 star on a

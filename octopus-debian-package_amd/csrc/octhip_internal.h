@@ -179,6 +179,10 @@ struct octhip_hamiltonian_s {
     hipStream_t s;
     int work_ldf, y_ldf;
     double *partial, *proj, *ksum, *y0, *y1, *y2;
+    // density of the propagated states: per-lane-chunk partials (dchunks x np) and the
+    // state weights (created on first use)
+    double *dpart, *dw;
+    int dchunks;
   };
   std::vector<WorkSet> sets;
   long graph_sets = 0;
@@ -193,6 +197,15 @@ struct octhip_hamiltonian_s {
   int y_ldf = -1;
   std::vector<void*> retired;   // work buffers outgrown by a larger ldf, freed at destroy
   int use_mfma = 1;
+  // Lanczos / Chebyshev buffers per stream: Krylov vectors (np_part rows each) and a small
+  // block of per-state factors and dot-product partials
+  struct KrylovSet {
+    hipStream_t s;
+    double* vec;
+    size_t bytes;
+    double* small;
+  };
+  std::vector<KrylovSet> krylov;
 };
 
 namespace octhip {
@@ -218,6 +231,20 @@ int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
 hipStream_t pin_workset(octhip_hamiltonian_s* h, hipStream_t s);
 // free the work-buffer set kept under key s (no synchronisation: nothing may use it)
 void drop_workset(octhip_hamiltonian_s* h, hipStream_t s);
+// free the Lanczos/Chebyshev buffers of stream s (every stream when all)
+void drop_krylov(octhip_hamiltonian_s* h, hipStream_t s, bool all);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s
 int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s);
+// exponential_apply_batch with its optional arguments: psi2/deltat2 (psib2), inh (inh_psib)
+// and the density of the propagated states (weights: host, one per complex state of the
+// row; density: device, np doubles, accumulated)
+struct TaylorOptions {
+  double* psi2 = nullptr;
+  double deltat2 = 0.0;
+  double* inh = nullptr;
+  const double* weights = nullptr;
+  double* density = nullptr;
+};
+int taylor_apply(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                 const TaylorOptions& o, hipStream_t s);
 }  // namespace octhip

@@ -56,6 +56,8 @@ template <>
 struct ZRings<EPI_TAYLOR> {
   static constexpr int own = 5, halo = 1, aux = 1, ring = 10;
 };
+template <>
+struct ZRings<EPI_TAYLOR_X> : ZRings<EPI_TAYLOR> {};
 
 // Raw buffer resources over the row arrays (gfx9 range checking, stride 0): a load whose
 // byte offset lies at or beyond num_records returns zeros and such a store is dropped.
@@ -135,7 +137,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const int col = threadIdx.x / PAIRS;
   const int cx = col % ZTX, cy = col / ZTX;
   const int l = chunk * LC + kp * LPT;         // first lane of this thread
-  const bool has_acc = (EPI == EPI_TAYLOR) && a_acc != nullptr;
+  constexpr bool TAY = EPI != EPI_STORE;
+  const bool has_acc = TAY && a_acc != nullptr;
   // row q of an array starts at byte q << sh; arrays without a role here get no records
   const int sh = __builtin_ctz((unsigned)A.L) + 3;
   const uint32_t lbyte = (uint32_t)l * 8u;
@@ -149,6 +152,10 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   const auto r_acc = zrsrc(a_acc, has_acc ? np_bytes : 0u);
   const auto r_ya = zrsrc(A.ya, (has_acc && A.ya) ? np_bytes : 0u);  // earlier powers, last order
   const auto r_yb = zrsrc(A.yb, (has_acc && A.yb) ? np_bytes : 0u);
+  // EPI_TAYLOR_X: second accumulator and density partials of this thread's state
+  const auto r_a2 = zrsrc(A.acc2, (EPI == EPI_TAYLOR_X && has_acc && A.acc2) ? np_bytes : 0u);
+  double* const a_dpart = EPI == EPI_TAYLOR_X ? A.dpart : nullptr;
+  const double wl = (EPI == EPI_TAYLOR_X && A.dw) ? A.dw[l >> 1] : 0.0;
   auto roff = [&](int q) { return ((uint32_t)q << sh) + lbyte; };
   // halo units of this thread: halo site h = col + ZNC*i (if h < ZNH), same lane pair
   int hlds[ZHU];
@@ -236,7 +243,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   double vr[AR] = {};
   Vec<LPT> kr[AR];
   Vec<LPT> ar[AR];
-  Vec<LPT> ra[EPI == EPI_TAYLOR ? AR : 1], rb[EPI == EPI_TAYLOR ? AR : 1];
+  Vec<LPT> ra[TAY ? AR : 1], rb[TAY ? AR : 1];
   // Every global load below is issued unconditionally (out-of-range requests read zeros):
   // with loads behind branches the compiler cannot count the loads still in flight and
   // drains the whole prefetch queue at every plane.
@@ -250,7 +257,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
       const int ks = tks[(P - zb + 4) * ZNC + col];
       kr[slot] = bld<LPT>(r_k, roff(qo >= 0 ? ks : -1));
     }
-    if constexpr (EPI == EPI_TAYLOR) {
+    if constexpr (TAY) {
       ar[slot] = bld<LPT>(r_ai, roff(qo));
       ra[slot] = bld<LPT>(r_ya, roff(qo));  // zeros (no memory access) without ya / yb
       rb[slot] = bld<LPT>(r_yb, roff(qo));
@@ -363,6 +370,43 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
               av.v[0] = av.v[0] + zmul_lane(a_c2r, a_c2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
             }
             bst<LPT>(r_acc, roff(q0), av);
+            if constexpr (EPI == EPI_TAYLOR_X) {
+              Vec<LPT> dv = av;
+              if (A.acc2) {
+                // psib2: the same terms with the deltat2 factors, from the same base
+                Vec<LPT> a2 = ar[kk % AR];
+                if constexpr (LPT == 2) {
+                  a2.v[0] = a2.v[0] + zmul_lane(A.dar, A.dai, ua.v[0], ua.v[1], false);
+                  a2.v[1] = a2.v[1] + zmul_lane(A.dar, A.dai, ua.v[1], ua.v[0], true);
+                  a2.v[0] = a2.v[0] + zmul_lane(A.dbr, A.dbi, ub.v[0], ub.v[1], false);
+                  a2.v[1] = a2.v[1] + zmul_lane(A.dbr, A.dbi, ub.v[1], ub.v[0], true);
+                  if (a_has_c1) {
+                    a2.v[0] = a2.v[0] + zmul_lane(A.d1r, A.d1i, x.v[0], x.v[1], false);
+                    a2.v[1] = a2.v[1] + zmul_lane(A.d1r, A.d1i, x.v[1], x.v[0], true);
+                  }
+                  a2.v[0] = a2.v[0] + zmul_lane(A.d2r, A.d2i, s.v[0], s.v[1], false);
+                  a2.v[1] = a2.v[1] + zmul_lane(A.d2r, A.d2i, s.v[1], s.v[0], true);
+                } else {
+                  const bool odd = (l & 1) != 0;
+                  a2.v[0] = a2.v[0] + zmul_lane(A.dar, A.dai, ua.v[0], __shfl_xor(ua.v[0], 1), odd);
+                  a2.v[0] = a2.v[0] + zmul_lane(A.dbr, A.dbi, ub.v[0], __shfl_xor(ub.v[0], 1), odd);
+                  if (a_has_c1) a2.v[0] = a2.v[0] + zmul_lane(A.d1r, A.d1i, x.v[0], __shfl_xor(x.v[0], 1), odd);
+                  a2.v[0] = a2.v[0] + zmul_lane(A.d2r, A.d2i, s.v[0], __shfl_xor(s.v[0], 1), odd);
+                }
+                bst<LPT>(r_a2, roff(q0), a2);
+                dv = a2;
+              }
+              if (a_dpart) {
+                // |value|^2 of this thread's lanes times its state's weight, summed over
+                // the PAIRS consecutive threads of the column (one lane chunk)
+                double t = dv.v[0] * dv.v[0];
+                if constexpr (LPT == 2) t = t + dv.v[1] * dv.v[1];
+                t = wl * t;
+#pragma unroll
+                for (int off = PAIRS >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off);
+                if (kp == 0) a_dpart[(size_t)chunk * a_np + q0] = t;
+              }
+            }
           }
         }
       }

@@ -15,6 +15,12 @@ int launch_nl(const ZArgs& z, int epi, int grid, size_t shmem, hipStream_t s) {
     } else {
       return fail(OCTHIP_ERR_INVALID, "z-march Taylor epilogue needs complex lane pairs");
     }
+  } else if (epi == EPI_TAYLOR_X) {
+    if constexpr (LC >= 2) {
+      hipLaunchKernelGGL((k_zmarch<LPT, LC, EPI_TAYLOR_X, NL>), dim3(grid), dim3(NT), shmem, s, z);
+    } else {
+      return fail(OCTHIP_ERR_INVALID, "z-march Taylor epilogue needs complex lane pairs");
+    }
   } else {
     hipLaunchKernelGGL((k_zmarch<LPT, LC, EPI_STORE, NL>), dim3(grid), dim3(NT), shmem, s, z);
   }
@@ -24,7 +30,7 @@ int launch_nl(const ZArgs& z, int epi, int grid, size_t shmem, hipStream_t s) {
 template <int LPT, int LC>
 int launch_lc(const ZArgs& z, int epi, int grid, int zseg, hipStream_t s) {
   const size_t shmem = zmarch_shmem(LC, zseg);
-  ProfScope prof("k_apply", s);
+  ProfScope prof("k_zmarch", s);
   const int rc = z.a.kslot ? launch_nl<LPT, LC, true>(z, epi, grid, shmem, s)
                          : launch_nl<LPT, LC, false>(z, epi, grid, shmem, s);
   if (rc) return rc;
@@ -36,9 +42,11 @@ template <int LPT, int LC>
 int launch_multi_lc(const ZArgs* zs, const int4* sched, int len, int epi, bool nl, int zseg, hipStream_t s) {
   constexpr int NT = ZNC * LC / LPT;
   const size_t shmem = zmarch_shmem(LC, zseg);
-  ProfScope prof("k_apply", s);
+  ProfScope prof("k_zmarch", s);
   if constexpr (NT > 1024) {
     return fail(OCTHIP_ERR_INVALID, "z-march: %d lanes per workgroup exceed 1024 threads", LC);
+  } else if (epi == EPI_TAYLOR_X) {
+    return fail(OCTHIP_ERR_UNSUPPORTED, "batched z-march: no psib2/density epilogue");
   } else if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
       if (nl) hipLaunchKernelGGL((k_zmarch_multi<LPT, LC, EPI_TAYLOR, true>), dim3(len), dim3(NT), shmem, s, zs, sched);
@@ -58,6 +66,14 @@ template <int LPT, int LC>
 const void* kernel_of(int epi, bool nl, bool multi) {
   if constexpr (ZNC * LC / LPT > 1024) {
     return nullptr;
+  } else if (epi == EPI_TAYLOR_X) {
+    if constexpr (LC >= 2) {
+      if (multi) return nullptr;
+      return nl ? reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_TAYLOR_X, true>)
+                : reinterpret_cast<const void*>(&k_zmarch<LPT, LC, EPI_TAYLOR_X, false>);
+    } else {
+      return nullptr;
+    }
   } else if (epi == EPI_TAYLOR) {
     if constexpr (LC >= 2) {
       if (multi)
@@ -81,10 +97,10 @@ const void* kernel_of(int epi, bool nl, bool multi) {
 // registers alone let run on a CU (LDS not counted): the segment-length choice keeps the
 // LDS of that many resident.  Cached per instance; 1 for shapes without an instance.
 int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi) {
-  static int cache[2][2][6][2][2] = {};  // [multi][lpt-1][log2 lc][epi][nl], 0 = not asked yet
+  static int cache[2][2][6][3][2] = {};  // [multi][lpt-1][log2 lc][epi][nl], 0 = not asked yet
   int lg = 0;
   while ((1 << lg) < lc) ++lg;
-  if (lpt < 1 || lpt > 2 || lg > 5 || (1 << lg) != lc || epi < 0 || epi > 1) return 1;
+  if (lpt < 1 || lpt > 2 || lg > 5 || (1 << lg) != lc || epi < 0 || epi > 2) return 1;
   int& c = cache[multi ? 1 : 0][lpt - 1][lg][epi][nl ? 1 : 0];
   if (c > 0) return c;
   const void* f = nullptr;

@@ -7,18 +7,21 @@
 !   int32 rimap_inv(nri+1); float64 v(np); int32 offsets(6*nmat); float64 matrices;
 !   int32 maps; float64 scals; float64 psi(L*np_part); float64 deltat; int32 order.
 ! output: float64 lap(L*np) (factor -1/2), hpsi(L*np), psi after one Taylor step
-!   (L*np_part).  Exit status 0 on success; on a library error the message of
-!   octhip_get_last_error is printed and the status is 2.
+!   (L*np_part), that psi unpacked on the device (L/2 complex states of np points, state
+!   after state: batch_read_device_to_unpacked) and packed again into a fresh batch (L*np).
+!   Exit status 0 on success; on a library error the message of octhip_get_last_error is
+!   printed and the status is 2.
 program octhip_driver
   use, intrinsic :: iso_c_binding
   use octhip_m
   implicit none
 
   integer(c_int) :: hdr(9), n, nri, np, np_part, ldf, nmat, msize, tpts, nproj, L, order
-  real(c_double), allocatable, target :: w(:), v(:), matrices(:), scals(:), psi(:), lap(:), hpsi(:)
+  real(c_double), allocatable, target :: w(:), v(:), matrices(:), scals(:), psi(:), lap(:), hpsi(:), &
+    unp(:), repacked(:)
   integer(c_int), allocatable, target :: points(:), ri(:), rimap_inv(:), offsets(:), maps(:)
   real(c_double) :: deltat
-  type(c_ptr) :: op, hm, d_psi, d_out, s
+  type(c_ptr) :: op, hm, d_psi, d_out, d_unp, s
   character(len=4096) :: fin, fout
   integer :: u
 
@@ -35,7 +38,7 @@ program octhip_driver
   L = ishft(1, ldf)
   allocate(w(n), points(3 * n), ri(n * nri), rimap_inv(nri + 1), v(np), offsets(max(1, 6 * nmat)), &
     matrices(max(1, msize)), maps(max(1, tpts)), scals(max(1, nproj)), psi(L * np_part), lap(L * np), &
-    hpsi(L * np))
+    hpsi(L * np), unp(L * np), repacked(L * np))
   read(u) w, points, ri, rimap_inv, v
   if (nmat > 0) read(u) offsets(1:6 * nmat), matrices(1:msize), maps(1:tpts), scals(1:nproj)
   read(u) psi, deltat, order
@@ -67,16 +70,26 @@ program octhip_driver
   ! one Taylor step in place
   call check(octhip_exponential_taylor_batch(hm, d_psi, ldf, deltat, order, 0_c_int, s), 'taylor')
   call check(octhip_memcpy_dtoh(c_loc(psi), d_psi, int(8, c_size_t) * L * np_part, s), 'dtoh psi')
+
+  ! do_unpack / do_pack on the device: the propagated batch to one column per state and back
+  call check(octhip_malloc(d_unp, int(8, c_size_t) * L * np), 'malloc unpacked')
+  call check(octhip_batch_unpack(d_psi, ldf, 0_c_int, L / 2, np, 1_c_int, d_unp, int(np, c_size_t), s), &
+    'batch_unpack')
+  call check(octhip_memcpy_dtoh(c_loc(unp), d_unp, int(8, c_size_t) * L * np, s), 'dtoh unpacked')
+  call check(octhip_batch_pack(d_unp, int(np, c_size_t), L / 2, np, 1_c_int, d_out, ldf, 0_c_int, s), &
+    'batch_pack')
+  call check(octhip_memcpy_dtoh(c_loc(repacked), d_out, int(8, c_size_t) * L * np, s), 'dtoh repacked')
   call check(octhip_stream_synchronize(s), 'synchronize')
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-  write(u) lap, hpsi, psi
+  write(u) lap, hpsi, psi, unp, repacked
   close(u)
 
   call check(octhip_hamiltonian_destroy(hm), 'hamiltonian_destroy')
   call check(octhip_nl_operator_destroy(op), 'nl_operator_destroy')
   call check(octhip_free(d_psi), 'free')
   call check(octhip_free(d_out), 'free')
+  call check(octhip_free(d_unp), 'free')
   call check(octhip_finalize(), 'finalize')
 
 contains

@@ -30,6 +30,8 @@ module octhip_m
     octhip_hamiltonian_apply_batch, octhip_exponential_taylor_batch, &
     octhip_taylor_graph_create, octhip_taylor_graph_launch, octhip_taylor_graph_destroy, &
     octhip_adjoint_create, octhip_adjoint_forward, octhip_adjoint_backward, octhip_adjoint_destroy, &
+    octhip_batch_pack, octhip_batch_unpack, octhip_exponential_apply_batch, octhip_density_accumulate, &
+    octhip_propagator_expmid, octhip_propagator_etrs, &
     octhip_last_error_message
 
   interface
@@ -151,6 +153,55 @@ module octhip_m
       type(c_ptr), value :: hm, psi, stream
       integer(c_int), value :: ldf, order, imag_time
       real(c_double), value :: deltat
+    end function
+    ! exponential_apply_batch(te, ..., psib, deltat, psib2, deltat2, imag_time=, inh_psib=) with
+    ! the density of the result (propagation_ops_elec_fuse_density_exp_apply); pass
+    ! c_null_ptr for absent optional batches and for weights/density
+    integer(c_int) function octhip_exponential_apply_batch(hm, psi, ldf, deltat, order, imag_time, psi2, &
+        deltat2, inh, weights, density, stream) bind(C, name="octhip_exponential_apply_batch")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, psi, psi2, inh, weights, density, stream
+      integer(c_int), value :: ldf, order, imag_time
+      real(c_double), value :: deltat, deltat2
+    end function
+    ! density_calc_accumulate on a device-packed batch (weights and density on the device)
+    integer(c_int) function octhip_density_accumulate(psi, ldf, np, is_complex, weights, density, stream) &
+        bind(C, name="octhip_density_accumulate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: psi, weights, density, stream
+      integer(c_int), value :: ldf, np, is_complex
+    end function
+    ! exponential_midpoint / td_etrs for independent particles over the batches psi(1:nbatch)
+    integer(c_int) function octhip_propagator_expmid(hm, psi, nbatch, ldf, dt, order, field_mid, weights, &
+        density, stream) bind(C, name="octhip_propagator_expmid")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, field_mid, weights, density, stream
+      type(c_ptr), intent(in) :: psi(*)
+      integer(c_int), value :: nbatch, ldf, order
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function octhip_propagator_etrs(hm, psi, nbatch, ldf, dt, order, field_old, field_new, &
+        weights, density, stream) bind(C, name="octhip_propagator_etrs")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, field_old, field_new, weights, density, stream
+      type(c_ptr), intent(in) :: psi(*)
+      integer(c_int), value :: nbatch, ldf, order
+      real(c_double), value :: dt
+    end function
+    ! batch_write_unpacked_to_device / batch_read_device_to_unpacked (batch.F90:625-760)
+    integer(c_int) function octhip_batch_pack(unpacked, ld, nst, np, is_complex, packed, ldf, first, stream) &
+        bind(C, name="octhip_batch_pack")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: unpacked, packed, stream
+      integer(c_size_t), value :: ld
+      integer(c_int), value :: nst, np, is_complex, ldf, first
+    end function
+    integer(c_int) function octhip_batch_unpack(packed, ldf, first, nst, np, is_complex, unpacked, ld, stream) &
+        bind(C, name="octhip_batch_unpack")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: packed, unpacked, stream
+      integer(c_size_t), value :: ld
+      integer(c_int), value :: ldf, first, nst, np, is_complex
     end function
     integer(c_int) function octhip_taylor_graph_create(hm, psi, nbatch, ldf, deltat, order, imag_time, &
         stream, graph) bind(C, name="octhip_taylor_graph_create")

@@ -36,6 +36,8 @@ SIGNATURES = {
     "octhip_memcpy_dtoh": [_P, _P, _S, _P],
     "octhip_memcpy_dtod": [_P, _P, _S, _P],
     "octhip_stream_synchronize": [_P],
+    "octhip_batch_pack": [_P, _S, _I, _I, _I, _P, _I, _I, _P],
+    "octhip_batch_unpack": [_P, _I, _I, _I, _I, _I, _P, _S, _P],
     "octhip_nl_operator_create": [_I, _DP, _IP, _I, _IP, _IP, _I, _I, _IP, C.POINTER(_P)],
     "octhip_nl_operator_destroy": [_P],
     "octhip_nl_operator_uses_lattice": [_P, _IP],
@@ -48,6 +50,15 @@ SIGNATURES = {
     "octhip_hamiltonian_set_dipole_field": [_P, _DP, _DP, _P],
     "octhip_hamiltonian_apply_batch": [_P, _P, _P, _I, _I, _I, _I, _P],
     "octhip_exponential_taylor_batch": [_P, _P, _I, _D, _I, _I, _P],
+    "octhip_exponential_apply_batch": [_P, _P, _I, _D, _I, _I, _P, _D, _P, _DP, _P, _P],
+    "octhip_density_accumulate": [_P, _I, _I, _I, _P, _P, _P],
+    "octhip_mesh_batch_dotp_vector": [_P, _P, _I, _I, _I, _D, _DP, _P],
+    "octhip_mesh_batch_nrm2": [_P, _I, _I, _I, _D, _DP, _P],
+    "octhip_mesh_batch_orthogonalization": [_I, _P, _P, _I, _I, _I, _D, _I, _DP, _DP, _P],
+    "octhip_exponential_lanczos_batch": [_P, _P, _I, _I, _D, _I, _I, _D, _P, _D, _IP, _DP, _P],
+    "octhip_exponential_cheby_batch": [_P, _P, _I, _D, _I, _D, _D, _P],
+    "octhip_propagator_expmid": [_P, _P, _I, _I, _D, _I, _DP, _DP, _P, _P],
+    "octhip_propagator_etrs": [_P, _P, _I, _I, _D, _I, _DP, _DP, _DP, _P, _P],
     "octhip_taylor_graph_create": [_P, _P, _I, _I, _D, _I, _I, _P, _P],
     "octhip_taylor_graph_launch": [_P, _P],
     "octhip_taylor_graph_destroy": [_P],

@@ -80,6 +80,31 @@ class Batch:
         return Batch(self.nst, self.nrows, self.is_complex, self.pack_size, ff=self.ff.clone())
 
 
+def batch_pack(unpacked, psib: "Batch", first: int = 0) -> None:
+    """batch_write_unpacked_to_device (src/grid/batch.F90:625-701): unpacked is a torch
+    tensor (nst, ld) on the device, complex128 for complex batches (float64 otherwise),
+    one state per row as batch_t%zff_linear's columns; states go to elements
+    first..first+nst-1 of psib's rows 0..np-1 (np = min(ld, psib.nrows))."""
+    torch = _torch()
+    want = torch.complex128 if psib.is_complex else torch.float64
+    if unpacked.dtype != want or not unpacked.is_cuda or not unpacked.is_contiguous() or unpacked.dim() != 2:
+        raise ValueError("unpacked must be a contiguous (nst, ld) device tensor of the batch's type")
+    nst, ld = unpacked.shape
+    call("octhip_batch_pack", C.c_void_p(unpacked.data_ptr()), ld, nst, min(ld, psib.nrows), int(psib.is_complex),
+         psib.ptr(), psib.ldf, int(first), _stream())
+
+
+def batch_unpack(psib: "Batch", unpacked, first: int = 0) -> None:
+    """batch_read_device_to_unpacked (src/grid/batch.F90:703-760): the reverse of batch_pack."""
+    torch = _torch()
+    want = torch.complex128 if psib.is_complex else torch.float64
+    if unpacked.dtype != want or not unpacked.is_cuda or not unpacked.is_contiguous() or unpacked.dim() != 2:
+        raise ValueError("unpacked must be a contiguous (nst, ld) device tensor of the batch's type")
+    nst, ld = unpacked.shape
+    call("octhip_batch_unpack", psib.ptr(), psib.ldf, int(first), nst, min(ld, psib.nrows), int(psib.is_complex),
+         C.c_void_p(unpacked.data_ptr()), ld, _stream())
+
+
 class NLOperatorDevice:
     """nl_operator_t on the device, built from the host arrays of nl_operator_build."""
 
@@ -154,11 +179,24 @@ class HamiltonianDevice:
              as_ptr(scals, C.c_double), C.byref(h))
         self.handle = h
 
+    # mesh volume element (the Lanczos inner products) and the spectral range the Chebyshev
+    # expansion assumes (hm%spectral_middle_point / spectral_half_span)
+    vol_pp = None
+    spectral_middle_point = None
+    spectral_half_span = None
+
     @classmethod
     def from_system(cls, system, lattice: bool = True, mass: float = 1.0) -> "HamiltonianDevice":
         from .stencil import star_points
         kin = NLOperatorDevice(system.op, star_points(4) if lattice else None)
-        return cls(kin, system.v, system.proj, mass)
+        hm = cls(kin, system.v, system.proj, mass)
+        hm.vol_pp = float(system.mesh.vol_pp)
+        return hm
+
+    def span(self, delta: float, emin: float) -> None:
+        """hamiltonian_elec_span (src/hamiltonian/hamiltonian_elec.F90:898-909)."""
+        self.spectral_middle_point = ((np.pi ** 2 / (2 * delta ** 2)) + emin) / 2.0
+        self.spectral_half_span = ((np.pi ** 2 / (2 * delta ** 2)) - emin) / 2.0
 
     def tune(self, **knobs) -> None:
         """bra_mode (0 plain, 1 FP64 MFMA) plus the kinetic operator's knobs."""
@@ -202,17 +240,179 @@ def hamiltonian_elec_apply_batch(hm: HamiltonianDevice, psib: Batch, hpsib: Batc
          int(psib.is_complex), int(terms), int(bool(set_bc)), _stream())
 
 
+def _ptr_or_none(b):
+    return None if b is None else b.ptr()
+
+
 class Exponential:
-    """exponential_t with exp_method = taylor (TDExpOrder default 4, share/varinfo)."""
+    """exponential_t (src/electrons/exponential.F90:45-194): exp_method taylor (TDExpOrder
+    default 4), lanczos (TDLanczosTol default 1e-5) or chebyshev."""
 
-    def __init__(self, exp_order: int = 4):
+    def __init__(self, exp_order: int = 4, method: str = "taylor", lanczos_tol: float = 1e-5):
+        if method not in ("taylor", "lanczos", "chebyshev"):
+            raise ValueError(f"unknown exponential method {method}")
         self.exp_order = exp_order
+        self.exp_method = method
+        self.lanczos_tol = lanczos_tol
+        self.last_iterations = None  # Lanczos: Krylov dimension and error estimate of the last call
+        self.last_residual = None
 
-    def apply_batch(self, hm: HamiltonianDevice, psib: Batch, deltat: float, imag_time: bool = False) -> None:
+    def _lanczos(self, hm, psib, deltat, imag_time, inh_psib=None):
+        if hm.vol_pp is None:
+            raise ValueError("Lanczos needs the mesh volume element (hm.vol_pp)")
+        it, res = C.c_int(), C.c_double()
+        call("octhip_exponential_lanczos_batch", hm.handle, psib.ptr(), psib.ldf, psib.nst, float(deltat),
+             int(bool(imag_time)), self.exp_order, float(self.lanczos_tol), _ptr_or_none(inh_psib),
+             float(hm.vol_pp), C.byref(it), C.byref(res), _stream())
+        self.last_iterations, self.last_residual = it.value, res.value
+        if res.value > self.lanczos_tol:
+            import warnings
+            warnings.warn(f"Lanczos exponential expansion did not converge: {res.value:.2e}")
+
+    def _cheby(self, hm, psib, deltat):
+        if hm.spectral_half_span is None:
+            raise ValueError("Chebyshev needs the spectral range (hm.span(delta, emin))")
+        call("octhip_exponential_cheby_batch", hm.handle, psib.ptr(), psib.ldf, float(deltat), self.exp_order,
+             float(hm.spectral_middle_point), float(hm.spectral_half_span), _stream())
+
+    def _apply_krylov(self, hm, psib, deltat, imag_time, psib2, deltat2, inh_psib):
+        """The lanczos and chebyshev branches of exponential_apply_batch
+        (src/electrons/exponential.F90:648-669)."""
+        if psib2 is not None:
+            call("octhip_memcpy_dtod", psib2.ptr(), psib.ptr(), hm.np * psib.lanes * 8, _stream())
+        if self.exp_method == "lanczos":
+            self._lanczos(hm, psib, deltat, imag_time)
+            if inh_psib is not None:
+                self._lanczos(hm, psib, deltat, imag_time, inh_psib)
+            if psib2 is not None:
+                self._lanczos(hm, psib2, deltat2, imag_time)
+                if inh_psib is not None:
+                    self._lanczos(hm, psib2, deltat2, imag_time, inh_psib)
+            return
+        if imag_time:
+            raise ValueError("imaginary time needs the taylor or lanczos method")  # exponential.F90:614-618
+        if inh_psib is not None:
+            raise ValueError("the Chebyshev exponential has no inhomogeneous term")  # exponential.F90:660-664
+        self._cheby(hm, psib, deltat)
+        if psib2 is not None:
+            self._cheby(hm, psib2, deltat2)
+
+    def apply_batch(self, hm: HamiltonianDevice, psib: Batch, deltat: float, imag_time: bool = False, *,
+                    psib2: Batch | None = None, deltat2: float | None = None, inh_psib: Batch | None = None,
+                    density=None, weights=None) -> None:
+        """te%apply_batch(namespace, mesh, hm, psib, deltat, psib2, deltat2, vmagnus, imag_time,
+        inh_psib) (src/electrons/exponential.F90:581-592); density (torch float64 (np,) on the
+        device) += sum_st weights[st] |final psi|^2 of psib2 when given, else psib (the
+        density_calc_accumulate of propagation_ops_elec_fuse_density_exp_apply)."""
         if not psib.is_complex:
             raise ValueError("exponential_apply_batch needs a complex batch")
-        call("octhip_exponential_taylor_batch", hm.handle, psib.ptr(), psib.ldf, float(deltat),
-             self.exp_order, int(bool(imag_time)), _stream())
+        if (psib2 is None) != (deltat2 is None):
+            raise ValueError("psib2 and deltat2 go together")  # ASSERT at exponential.F90:602
+        for b in (psib2, inh_psib):
+            if b is not None and (b.ldf != psib.ldf or not b.is_complex):
+                raise ValueError("psib2 / inh_psib must match psib")
+        if (density is None) != (weights is None):
+            raise ValueError("density and weights go together")
+        if self.exp_method != "taylor":
+            self._apply_krylov(hm, psib, deltat, imag_time, psib2, deltat2, inh_psib)
+            if density is not None:
+                density_calc_accumulate(psib2 if psib2 is not None else psib, hm.np, weights, density)
+            return
+        if psib2 is None and inh_psib is None and density is None:
+            call("octhip_exponential_taylor_batch", hm.handle, psib.ptr(), psib.ldf, float(deltat),
+                 self.exp_order, int(bool(imag_time)), _stream())
+            return
+        w = None if weights is None else _weights(weights, psib.pack_size)
+        call("octhip_exponential_apply_batch", hm.handle, psib.ptr(), psib.ldf, float(deltat), self.exp_order,
+             int(bool(imag_time)), _ptr_or_none(psib2), float(deltat2 or 0.0), _ptr_or_none(inh_psib),
+             as_ptr(w, C.c_double), None if density is None else C.c_void_p(density.data_ptr()), _stream())
+
+
+def _weights(weights, pack: int) -> np.ndarray:
+    """kweight*occ per state, padded with zeros to the pack size."""
+    w = np.zeros(pack)
+    v = np.asarray(weights, dtype=np.float64).ravel()
+    if len(v) > pack:
+        raise ValueError("more weights than states in the batch")
+    w[: len(v)] = v
+    return w
+
+
+def mesh_batch_dotp_vector(vol: float, aa: Batch, bb: Batch, rows: int) -> np.ndarray:
+    """<aa_st|bb_st> = vol sum_p conj(aa) bb per state (src/grid/mesh_batch_inc.F90:379-549)."""
+    if aa.lanes != bb.lanes or aa.is_complex != bb.is_complex:
+        raise ValueError("batches are not compatible")
+    n = aa.pack_size if aa.is_complex else aa.lanes
+    out = np.zeros(2 * n if aa.is_complex else n)
+    call("octhip_mesh_batch_dotp_vector", aa.ptr(), bb.ptr(), aa.ldf, int(rows), int(aa.is_complex), float(vol),
+         as_ptr(out, C.c_double), _stream())
+    return (out.view(np.complex128) if aa.is_complex else out)[: aa.nst].copy()
+
+
+def mesh_batch_nrm2(vol: float, aa: Batch, rows: int) -> np.ndarray:
+    """mesh_batch_nrm2 (src/grid/mesh_batch.F90:74-95)."""
+    n = aa.pack_size if aa.is_complex else aa.lanes
+    out = np.zeros(n)
+    call("octhip_mesh_batch_nrm2", aa.ptr(), aa.ldf, int(rows), int(aa.is_complex), float(vol),
+         as_ptr(out, C.c_double), _stream())
+    return out[: aa.nst].copy()
+
+
+def mesh_batch_orthogonalization(vol: float, psibs: list, phib: Batch, rows: int, normalize: bool = False):
+    """zmesh_batch_orthogonalization, classical Gram-Schmidt (src/grid/mesh_batch_inc.F90:1168-1274).
+    Returns (overlap (len(psibs), nst) complex, norm (nst,))."""
+    n = len(psibs)
+    ptrs = (C.c_void_p * max(n, 1))(*[b.ptr() for b in psibs])
+    ov = np.zeros(2 * n * phib.nst)
+    nrm = np.zeros(phib.nst)
+    call("octhip_mesh_batch_orthogonalization", n, ptrs, phib.ptr(), phib.ldf, phib.nst, int(rows), float(vol),
+         int(bool(normalize)), as_ptr(ov, C.c_double), as_ptr(nrm, C.c_double), _stream())
+    return ov.view(np.complex128).reshape(phib.nst, n).T.copy(), nrm
+
+
+def density_calc_accumulate(psib: Batch, rows: int, weights, density) -> None:
+    """density_calc_accumulate on a device-packed batch (src/states/density.F90:279-424):
+    density (torch float64 on the device) += sum_st weights[st] |psi_st|^2 over rows points."""
+    torch = _torch()
+    n = psib.pack_size if psib.is_complex else psib.lanes
+    w = torch.from_numpy(_weights(weights, n)).to("cuda")
+    call("octhip_density_accumulate", psib.ptr(), psib.ldf, int(rows), int(psib.is_complex),
+         C.c_void_p(w.data_ptr()), C.c_void_p(density.data_ptr()), _stream())
+    torch.cuda.current_stream().synchronize()  # w is freed on return
+
+
+def _driver_args(batches, weights):
+    ldf = batches[0].ldf
+    for b in batches:
+        if not b.is_complex or b.ldf != ldf:
+            raise ValueError("propagator batches must be complex with one ldf")
+    ptrs = (C.c_void_p * len(batches))(*[b.ptr() for b in batches])
+    w = None
+    if weights is not None:
+        w = np.concatenate([_weights(wb, batches[0].pack_size) for wb in weights])
+    return ldf, ptrs, w
+
+
+def propagator_expmid(te: Exponential, hm: HamiltonianDevice, batches: list, dt: float, field_mid=None,
+                      weights=None, density=None) -> None:
+    """exponential_midpoint for independent particles (src/td/propagator_expmid.F90:52-100);
+    weights: one list of state weights per batch; density (np,) on the device is set."""
+    ldf, ptrs, w = _driver_args(batches, weights)
+    f = None if field_mid is None else np.ascontiguousarray(field_mid, dtype=np.float64)
+    call("octhip_propagator_expmid", hm.handle, ptrs, len(batches), ldf, float(dt), te.exp_order,
+         as_ptr(f, C.c_double), as_ptr(w, C.c_double),
+         None if density is None else C.c_void_p(density.data_ptr()), _stream())
+
+
+def propagator_etrs(te: Exponential, hm: HamiltonianDevice, batches: list, dt: float, field_old=None,
+                    field_new=None, weights=None, density=None) -> None:
+    """td_etrs for independent particles (src/td/propagator_etrs.F90:68-150)."""
+    ldf, ptrs, w = _driver_args(batches, weights)
+    fo = None if field_old is None else np.ascontiguousarray(field_old, dtype=np.float64)
+    fn = None if field_new is None else np.ascontiguousarray(field_new, dtype=np.float64)
+    call("octhip_propagator_etrs", hm.handle, ptrs, len(batches), ldf, float(dt), te.exp_order,
+         as_ptr(fo, C.c_double), as_ptr(fn, C.c_double), as_ptr(w, C.c_double),
+         None if density is None else C.c_void_p(density.data_ptr()), _stream())
 
 
 class TaylorGraph:
@@ -249,8 +449,8 @@ class TaylorGraph:
 
 
 def exponential_apply_batch(te: Exponential, hm: HamiltonianDevice, psib: Batch, deltat: float,
-                            imag_time: bool = False) -> None:
-    te.apply_batch(hm, psib, deltat, imag_time)
+                            imag_time: bool = False, **optional) -> None:
+    te.apply_batch(hm, psib, deltat, imag_time=imag_time, **optional)
 
 
 class Adjoint:

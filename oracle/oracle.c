@@ -196,4 +196,107 @@ int oracle_taylor(const oracle_ham_t* h, double* psi, int ldf, double deltat, in
   return rc;
 }
 
-int oracle_version(void) { return 1; }
+/* batch_axpy with a real factor on the inner rows (src/grid/batch_ops_inc.F90:23-106,
+ * the real-constant branch used for real(zfact)): y += a * x on every double. */
+void oracle_daxpy(int np, int ldf, double a, const double* x, double* y) {
+  const size_t count = (size_t)np << ldf;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < count; ++i) y[i] = y[i] + a * x[i];
+}
+
+/* exponential_taylor_series_batch with its optional arguments
+ * (src/electrons/exponential.F90:677-757):
+ *   psi2 (optional): copy of psi's inner rows first (700), then receives the series at
+ *     deltat2 with the same H^k terms (740-743);
+ *   inh (optional): zfact = deltat, psi += real(zfact) inh (705-708; psi2 likewise with
+ *     deltat2, 710-713), the chain starts from inh (729-731) and the denominators are
+ *     k + 1 (718-719): psi += deltat sum_k (-i deltat H)^k / (k+1)! inh.
+ * Complex factors multiply as in oracle_taylor; every H-apply zeroes its input's
+ * boundary rows (set_bc default), inh's included. */
+static int taylor_series(const oracle_ham_t* h, double* psi, int ldf, double dtr, double dti, int order,
+                         double* psi2, double dt2r, double dt2i, double* inh) {
+  const size_t L = (size_t)1 << ldf;
+  const size_t rows = (size_t)h->np_part;
+  double* psi1 = (double*)calloc(rows * L, sizeof(double));
+  double* hpsi1 = (double*)calloc(rows * L, sizeof(double));
+  if (!psi1 || !hpsi1) {
+    free(psi1);
+    free(hpsi1);
+    return -1;
+  }
+  if (psi2) memcpy(psi2, psi, sizeof(double) * (size_t)h->np * L);
+  double zr = 1.0, zi = 0.0, z2r = 1.0, z2i = 0.0;
+  if (inh) {
+    zr = dtr;
+    zi = dti;
+    oracle_daxpy(h->np, ldf, zr, inh, psi);
+    if (psi2) {
+      z2r = dt2r;
+      z2i = dt2i;
+      oracle_daxpy(h->np, ldf, z2r, inh, psi2);
+    }
+  }
+  const double mr = -0.0, mi = -1.0;
+  const double sr = mr * dtr - mi * dti, si = mr * dti + mi * dtr;
+  const double s2r = mr * dt2r - mi * dt2i, s2i = mr * dt2i + mi * dt2r;
+  int rc = 0;
+  for (int k = 1; k <= order && rc == 0; ++k) {
+    const int denom = inh ? k + 1 : k;
+    const double tr = zr * sr - zi * si, ti = zr * si + zi * sr;
+    zr = tr / denom;
+    zi = ti / denom;
+    if (psi2) {
+      const double t2r = z2r * s2r - z2i * s2i, t2i = z2r * s2i + z2i * s2r;
+      z2r = t2r / denom;
+      z2i = t2i / denom;
+    }
+    double* src = k > 1 ? psi1 : (inh ? inh : psi);
+    rc = oracle_hamiltonian_apply(h, src, hpsi1, ldf, ORACLE_TERM_ALL, 1);
+    oracle_zaxpy(h->np, ldf, zr, zi, hpsi1, psi);
+    if (psi2) oracle_zaxpy(h->np, ldf, z2r, z2i, hpsi1, psi2);
+    if (k != order) memcpy(psi1, hpsi1, sizeof(double) * (size_t)h->np * L);
+  }
+  free(psi1);
+  free(hpsi1);
+  return rc;
+}
+
+/* exponential_apply_batch, Taylor branch (src/electrons/exponential.F90:581-674):
+ * deltat -> i deltat (and deltat2 -> i deltat2) in imaginary time; the homogeneous series
+ * (with psi2 when given), then, with inh, the inhomogeneous series (which starts by
+ * copying the propagated psi into psi2 again, 700). */
+int oracle_taylor_ex(const oracle_ham_t* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                     double* psi2, double deltat2, double* inh) {
+  const double dtr = imag_time ? 0.0 : deltat, dti = imag_time ? deltat : 0.0;
+  const double d2r = imag_time ? 0.0 : deltat2, d2i = imag_time ? deltat2 : 0.0;
+  int rc = taylor_series(h, psi, ldf, dtr, dti, order, psi2, d2r, d2i, NULL);
+  if (!rc && inh) rc = taylor_series(h, psi, ldf, dtr, dti, order, psi2, d2r, d2i, inh);
+  return rc;
+}
+
+/* density_calc_accumulate on a device-packed batch, kernels density_real /
+ * density_complex (src/states/density.F90:375-417, share/opencl/density.cl:24-61):
+ * dd = sum_ist weight(ist) * |psi(ist, p)|^2 in state order, density(p) += dd.  Complex
+ * batches: |z|^2 = re*re + im*im. */
+void oracle_density_accumulate(int np, int ldf, int nst, int is_complex, const double* weights,
+                               const double* psi, double* density) {
+  const size_t L = (size_t)1 << ldf;
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < np; ++p) {
+    double dd = 0.0;
+    for (int ist = 0; ist < nst; ++ist) {
+      double a2;
+      if (is_complex) {
+        const double re = psi[IDX(p, 2 * ist, L)], im = psi[IDX(p, 2 * ist + 1, L)];
+        a2 = re * re + im * im;
+      } else {
+        const double f = psi[IDX(p, ist, L)];
+        a2 = f * f;
+      }
+      dd = dd + weights[ist] * a2;
+    }
+    density[p] = density[p] + dd;
+  }
+}
+
+int oracle_version(void) { return 2; }

@@ -46,6 +46,11 @@ int oracle_hamiltonian_apply(const oracle_ham_t* h, double* psi, double* hpsi, i
                              int set_bc);
 void oracle_zaxpy(int np, int ldf, double zr, double zi, const double* x, double* y);
 int oracle_taylor(const oracle_ham_t* h, double* psi, int ldf, double deltat, int order, int imag_time);
+void oracle_daxpy(int np, int ldf, double a, const double* x, double* y);
+int oracle_taylor_ex(const oracle_ham_t* h, double* psi, int ldf, double deltat, int order, int imag_time,
+                     double* psi2, double deltat2, double* inh);
+void oracle_density_accumulate(int np, int ldf, int nst, int is_complex, const double* weights,
+                               const double* psi, double* density);
 int oracle_version(void);
 
 #ifdef __cplusplus

@@ -34,6 +34,11 @@ def lib():
         l = C.CDLL(str(LIB))
         l.oracle_taylor.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int,
                                     C.c_int]
+        l.oracle_taylor_ex.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int,
+                                       C.c_int, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
+        l.oracle_density_accumulate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        l.oracle_daxpy.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         l.oracle_hamiltonian_apply.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                C.c_int, C.c_int, C.c_int]
         _lib = l
@@ -107,6 +112,59 @@ class Hamiltonian:
                                  int(imag_time))
         if rc:
             raise RuntimeError("oracle_taylor failed")
+
+    def taylor_ex(self, psi: np.ndarray, deltat: float, order=4, imag_time=False, psi2=None, deltat2=0.0,
+                  inh=None) -> None:
+        """exponential_apply_batch with psib2/deltat2 and inh_psib (all (np_part, L), modified
+        in place; inh's boundary rows are zeroed by its H-apply, as in the reference)."""
+        L = psi.shape[1]
+        for a in (psi2, inh):
+            assert a is None or (a.shape == psi.shape and a.flags["C_CONTIGUOUS"])
+        rc = lib().oracle_taylor_ex(C.byref(self.s), _p(psi), int(np.log2(L)), C.c_double(deltat), order,
+                                    int(imag_time), _p(psi2), C.c_double(deltat2), _p(inh))
+        if rc:
+            raise RuntimeError("oracle_taylor_ex failed")
+
+
+def density_accumulate(psi: np.ndarray, rows: int, nst: int, is_complex: bool, weights, density: np.ndarray) -> None:
+    """density[:rows] += sum_ist weights[ist] |psi(ist, p)|^2 (the device kernels' order)."""
+    w = np.ascontiguousarray(weights, np.float64)
+    assert density.flags["C_CONTIGUOUS"] and density.dtype == np.float64 and len(w) >= nst
+    lib().oracle_density_accumulate(int(rows), int(np.log2(psi.shape[1])), int(nst), int(bool(is_complex)),
+                                    _p(w), _p(np.ascontiguousarray(psi)), _p(density))
+
+
+def exp_apply_batches(h: Hamiltonian, psis: list, dt: float, order: int = 4, weights=None):
+    """propagation_ops_elec_exp_apply / _fuse_density_exp_apply for independent particles
+    (src/td/propagation_ops_elec.F90:213-333): each packed complex batch propagated by dt in
+    turn; with weights (one array per batch, pack-size long), the density of the results
+    (density_calc_init zeroes it, density_calc_accumulate adds each batch)."""
+    rho = np.zeros(h.np) if weights is not None else None
+    for b, psi in enumerate(psis):
+        h.taylor(psi, dt, order)
+        if rho is not None:
+            density_accumulate(psi, h.np, psi.shape[1] // 2, True, weights[b], rho)
+    return rho
+
+
+def expmid_step(h: Hamiltonian, psis: list, dt: float, field_mid=None, order: int = 4, weights=None):
+    """exponential_midpoint (src/td/propagator_expmid.F90:52-100), independent particles:
+    H at t - dt/2 (the dipole field there), then every batch propagated by dt."""
+    if field_mid is not None:
+        h.set_field(field_mid)
+    return exp_apply_batches(h, psis, dt, order, weights)
+
+
+def etrs_step(h: Hamiltonian, psis: list, dt: float, field_old=None, field_new=None, order: int = 4,
+              weights=None):
+    """td_etrs (src/td/propagator_etrs.F90:68-150), independent particles: dt/2 with
+    H(t - dt), H updated to t, dt/2 with the density."""
+    if field_old is not None:
+        h.set_field(field_old)
+    exp_apply_batches(h, psis, 0.5 * dt, order)
+    if field_new is not None:
+        h.set_field(field_new)
+    return exp_apply_batches(h, psis, 0.5 * dt, order, weights)
 
 
 def operate_ri_vec(op, fi: np.ndarray, factor: float = 1.0) -> np.ndarray:
@@ -213,3 +271,108 @@ def overlap_objective(phi: np.ndarray, psi: np.ndarray, rows: int, vol: float):
     a, b = to_complex(phi, rows), to_complex(psi, rows)
     c = vol * np.sum(np.conj(a) * b, axis=0)
     return float(np.sum(np.abs(c) ** 2)), to_packed(a * c[None, :], phi.shape[0])
+
+
+# ------------------------------------------------------- Krylov exponentials --
+def mesh_dotp(a: np.ndarray, b: np.ndarray, rows: int, nst: int, vol: float) -> np.ndarray:
+    """<a_st|b_st> = vol sum_p conj(a) b for packed complex batches (mesh_batch_dotp_vector,
+    src/grid/mesh_batch_inc.F90:379-549)."""
+    za, zb = to_complex(a, rows)[:, :nst], to_complex(b, rows)[:, :nst]
+    return vol * np.sum(np.conj(za) * zb, axis=0)
+
+
+def _cols_axpy(y: np.ndarray, x: np.ndarray, coef, rows: int) -> None:
+    """y(:, st) += coef[st] x(:, st) on the inner rows of packed complex batches (states beyond
+    len(coef) untouched, batch_axpy with a_full = .false.)."""
+    zy = y[:rows].view(np.complex128)
+    zx = x[:rows].view(np.complex128)
+    n = len(coef)
+    zy[:, :n] += np.asarray(coef)[None, :] * zx[:, :n]
+
+
+def _cols_scal(x: np.ndarray, coef, rows: int) -> None:
+    zx = x[:rows].view(np.complex128)
+    n = len(coef)
+    zx[:, :n] = np.asarray(coef)[None, :] * zx[:, :n]
+
+
+def _hermitian_function(A: np.ndarray, pp: complex, phi: bool) -> np.ndarray:
+    """zlalg_exp / zlalg_phi, Hermitian branch (src/math/lalg_adv.F90:216-340): eigenpairs of
+    the matrix read from its upper triangle (zheev 'U'), f(pp A) = V f(pp D) V^H."""
+    w, V = np.linalg.eigh(A, UPLO="U")
+    x = pp * w
+    f = (np.exp(x) - 1.0) / x if phi else np.exp(x)
+    return (V * f[None, :]) @ np.conj(V).T
+
+
+def lanczos_batch(h: "Hamiltonian", psi: np.ndarray, nst: int, deltat: complex, order: int, tol: float,
+                  vol: float, inh: np.ndarray | None = None):
+    """exponential_lanczos_batch (src/electrons/exponential.F90:766-900) on a packed complex
+    batch, in place; Hermitian H (Lanczos: orthogonalisation against the last two vectors,
+    classical Gram-Schmidt).  Returns (Krylov dimension used, largest error estimate)."""
+    rows, L = h.np, psi.shape[1]
+    eps = np.finfo(np.float64).eps
+    vb = [np.zeros((h.np_part, L)) for _ in range(order + 1)]
+    vb[0][:rows] = (inh if inh is not None else psi)[:rows]
+    beta = np.sqrt(mesh_dotp(vb[0], vb[0], rows, nst, vol).real)
+    if np.all(np.abs(beta) <= 1e-12):
+        return 0, 0.0
+    _cols_scal(vb[0], 1.0 / beta, rows)
+    M = order + 1
+    hamilt = np.zeros((nst, M, M), np.complex128)
+    expo = np.zeros((nst, M), np.complex128)
+    pp = -1j * deltat
+    res = np.zeros(nst)
+    it = 1
+    while it <= order:
+        vb[it][:rows] = h.apply(vb[it - 1])
+        l = max(1, it - 1)
+        ss = [mesh_dotp(vb[j - 1], vb[it], rows, nst, vol) for j in range(l, it + 1)]
+        for j, o in zip(range(l, it + 1), ss):
+            hamilt[:, j - 1, it - 1] = o
+        for j, o in zip(range(l, it + 1), ss):
+            _cols_axpy(vb[it], vb[j - 1], -o, rows)
+        nrm = np.sqrt(mesh_dotp(vb[it], vb[it], rows, nst, vol).real)
+        hamilt[:, it, it - 1] = nrm
+        for st in range(nst):
+            expo[st, :it] = _hermitian_function(hamilt[st, :it, :it], pp, inh is not None)[:, 0]
+            res[st] = abs(hamilt[st, it, it - 1] * abs(expo[st, it - 1]))
+        if np.all(np.abs(hamilt[:, it, it - 1]) < 1e4 * eps):
+            break
+        sc = np.where(np.abs(hamilt[:, it, it - 1]) >= 1e4 * eps, 1.0 / np.abs(hamilt[:, it, it - 1]), 1.0)
+        _cols_scal(vb[it], sc, rows)
+        if it > 3 and np.all(res < tol):
+            break
+        it += 1
+    used = min(it, order)
+    if inh is not None:
+        for ii in range(1, used + 1):
+            _cols_axpy(psi, vb[ii - 1], (np.real(deltat) * beta) * expo[:, ii - 1], rows)
+    else:
+        _cols_scal(psi, expo[:, 0], rows)
+        for ii in range(2, used + 1):
+            _cols_axpy(psi, vb[ii - 1], beta * expo[:, ii - 1], rows)
+    return used, float(np.max(res))
+
+
+def cheby_batch(h: "Hamiltonian", psi: np.ndarray, deltat: float, order: int, mid: float, span: float) -> None:
+    """exponential_cheby_batch (src/electrons/exponential.F90:903-971), in place: Clenshaw's
+    recurrence for sum_j c_j T_j((H - mid)/span), c_j = 2 (-i)^j J_j(span deltat)."""
+    from scipy.special import jv
+    rows, L = h.np, psi.shape[1]
+    psi0 = np.zeros((h.np_part, L))
+    psi1 = np.zeros((h.np_part, L))
+    psi2 = np.zeros((h.np_part, L))
+    zb = to_complex(psi, rows)
+    for j in range(order - 1, -1, -1):
+        psi2[:rows] = psi1[:rows]
+        psi1[:rows] = psi0[:rows]
+        psi0[:rows] = h.apply(psi1)
+        zfact = 2.0 * (-1j) ** j * jv(j, span * deltat)
+        z0 = psi0[:rows].view(np.complex128)
+        z0 += -mid * psi1[:rows].view(np.complex128)
+        z0 *= 2.0 / span
+        z0 += zfact * zb
+        z0 += -1.0 * psi2[:rows].view(np.complex128)
+    out = psi0[:rows].view(np.complex128) - psi2[:rows].view(np.complex128)
+    psi[:rows] = (0.5 * np.exp(-1j * mid * deltat) * out).view(np.float64)

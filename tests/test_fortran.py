@@ -72,7 +72,13 @@ def test_fortran_driver_on_device(tmp_path):
     out = np.fromfile(tmp_path / "out.bin", np.float64)
     lap = out[: s.np * L].reshape(s.np, L)
     hpsi = out[s.np * L: 2 * s.np * L].reshape(s.np, L)
-    psi1 = out[2 * s.np * L:].reshape(s.np_part, L)
+    psi1 = out[2 * s.np * L: 2 * s.np * L + s.np_part * L].reshape(s.np_part, L)
+    rest = out[2 * s.np * L + s.np_part * L:]
+    unp = rest[: s.np * L].view(np.complex128).reshape(L // 2, s.np)
+    repacked = rest[s.np * L:].reshape(s.np, L)
+    # device unpack / pack: the host packer's layout, bit for bit
+    assert np.array_equal(unp, workloads.unpack(psi1, L // 2, s.np, True))
+    assert np.array_equal(repacked, psi1[: s.np])
     assert np.array_equal(lap, O.operate_ri_vec(s.op, psi, -0.5))
     oh = O.Hamiltonian(s.op, s.v, s.proj)
     assert max_rel_err(hpsi, oh.apply(psi.copy())) <= 1e-12
