@@ -61,7 +61,7 @@ __global__ __launch_bounds__(BLOCK) void k_unpack(const E* __restrict__ src, int
 
 int check_pack_args(const void* a, const void* b, size_t ld, int nst, int np, int is_complex, int ldf, int first) {
   OCTHIP_REQUIRE_INIT();
-  OCTHIP_REQUIRE(a && b && a != b, "bad buffer pointers");
+  OCTHIP_REQUIRE((nst == 0 || np == 0) || (a && b && a != b), "bad buffer pointers");
   OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);
   OCTHIP_REQUIRE(nst >= 0 && np >= 0 && ld >= (size_t)np, "bad sizes nst=%d np=%d ld=%zu", nst, np, ld);
   const int row = is_complex ? (1 << (ldf - 1)) : (1 << ldf);

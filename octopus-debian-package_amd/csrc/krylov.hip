@@ -20,7 +20,7 @@
 namespace octhip {
 size_t dotp_scratch_bytes(int L, int is_complex);
 int launch_dotp(const double* a, const double* b, int L, int np, int is_complex, double vol, double* scratch,
-                hipStream_t s, double** result);
+                hipStream_t s, double** result, double* dest = nullptr);
 int launch_axpy_cols(double* y, const double* x, int L, int np, int is_complex, const double* coef, hipStream_t s);
 int launch_scal_cols(double* x, int L, int np, int is_complex, const double* coef, hipStream_t s);
 int launch_cheby_update(double* out, const double* psi1, const double* b, const double* psi2, size_t count,

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -168,14 +169,14 @@ size_t dotp_scratch_bytes(int L, int is_complex) {
   return sizeof(double) * 2 * ((size_t)DOT_BLOCKS * c.n + c.n);
 }
 
-// dot[c] = vol * sum_p conj(a(p, c)) b(p, c) into scratch (partials, then the ncol complex
-// results at the end of scratch); returns the device address of the results.
+// dot[c] = vol * sum_p conj(a(p, c)) b(p, c): partials in scratch, the ncol complex results
+// in dest (device) or, when dest is null, at the end of scratch; *result = their address.
 int launch_dotp(const double* a, const double* b, int L, int np, int is_complex, double vol, double* scratch,
-                hipStream_t s, double** result) {
+                hipStream_t s, double** result, double* dest) {
   const Cols c = cols_of(L, is_complex);
   if (c.per > 4) return fail(OCTHIP_ERR_UNSUPPORTED, "dot product: more than %d columns", 4 * BLOCK);
   double* partial = scratch;
-  double* out = scratch + (size_t)2 * DOT_BLOCKS * c.n;
+  double* out = dest ? dest : scratch + (size_t)2 * DOT_BLOCKS * c.n;
   const int nblk = std::max(1, std::min(DOT_BLOCKS, (np + c.g - 1) / c.g));
   ProfScope prof("k_dotp", s);
   hipLaunchKernelGGL(k_dotp_partial, dim3(nblk), dim3(BLOCK), 0, s, a, b, L, np, is_complex, c.cw, c.per, partial);
@@ -235,7 +236,7 @@ extern "C" int octhip_mesh_batch_dotp_vector(const double* a, const double* b, i
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   double* res = nullptr;
-  rc = launch_dotp(a, b, L, np, is_complex, vol, scratch, s, &res);
+  rc = launch_dotp(a, b, L, np, is_complex, vol, scratch, s, &res, nullptr);
   const int n = is_complex ? L / 2 : L;
   std::vector<double> host(2 * (size_t)n);
   if (!rc) {
@@ -256,3 +257,4 @@ extern "C" int octhip_mesh_batch_dotp_vector(const double* a, const double* b, i
   }
   return OCTHIP_SUCCESS;
 }
+

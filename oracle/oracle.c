@@ -33,10 +33,14 @@ void oracle_operate_ri_vec(int n, const double* w, int nri, const int* ri, const
   for (int l = 0; l < nri; ++l) {
     const int* index = ri + (size_t)l * n;
     for (int i = imin[l]; i < imax[l]; ++i) {
-      for (int k = 0; k < L; ++k) {
-        double a = 0.0;
-        for (int j = 0; j < n; ++j) a = fma(w[j], fi[IDX(index[j] + i, k, L)], a);
-        fo[IDX(i, k, L)] = a;
+      /* lanes innermost (one row of accumulators, vectorisable as the leaf's VEC_FMA
+       * blocks are): every lane still sums j = 0..n-1 with fma from 0 */
+      double* out = fo + IDX(i, 0, L);
+      for (int k = 0; k < L; ++k) out[k] = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double wj = w[j];
+        const double* src = fi + IDX(index[j] + i, 0, L);
+        for (int k = 0; k < L; ++k) out[k] = fma(wj, src[k], out[k]);
       }
     }
   }
@@ -77,13 +81,18 @@ void oracle_nlocal_start(int nmat, const int* offsets, const double* matrices, c
   for (int a = 0; a < nmat; ++a) {
     const int* o = offsets + 6 * (size_t)a;
     const int npt = o[0], npj = o[1], moff = o[2], mapoff = o[3], soff = o[4];
+    /* blocks of up to 8 lanes innermost; each lane sums q = 0..npt-1 with fma from 0 */
+    const int kb = L < 8 ? L : 8;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int j = 0; j < npj; ++j)
-      for (int k = 0; k < L; ++k) {
-        double s = 0.0;
-        for (int q = 0; q < npt; ++q)
-          s = fma(psi[IDX(maps[mapoff + q] - 1, k, L)], matrices[moff + (size_t)j * npt + q], s);
-        proj[IDX(soff + j, k, L)] = s * scals[soff + j];
+      for (int k0 = 0; k0 < L; k0 += kb) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < npt; ++q) {
+          const double pj = matrices[moff + (size_t)j * npt + q];
+          const double* row = psi + IDX(maps[mapoff + q] - 1, k0, L);
+          for (int k = 0; k < kb; ++k) acc[k] = fma(row[k], pj, acc[k]);
+        }
+        for (int k = 0; k < kb; ++k) proj[IDX(soff + j, k0 + k, L)] = acc[k] * scals[soff + j];
       }
   }
 }
@@ -101,11 +110,16 @@ void oracle_nlocal_finish(int nmat, const int* offsets, const double* matrices, 
 #pragma omp parallel for schedule(static)
     for (int q = 0; q < npt; ++q) {
       const int p = maps[mapoff + q] - 1;
-      for (int k = 0; k < L; ++k) {
-        double t = 0.0;
-        for (int j = 0; j < npj; ++j) t = fma(matrices[moff + (size_t)j * npt + q], proj[IDX(soff + j, k, L)], t);
-        hpsi[IDX(p, k, L)] = hpsi[IDX(p, k, L)] + t;
+      /* lanes innermost; each lane sums j = 0..npj-1 with fma from 0, then adds */
+      double t[1024];
+      for (int k = 0; k < L; ++k) t[k] = 0.0;
+      for (int j = 0; j < npj; ++j) {
+        const double pq = matrices[moff + (size_t)j * npt + q];
+        const double* pr = proj + IDX(soff + j, 0, L);
+        for (int k = 0; k < L; ++k) t[k] = fma(pq, pr[k], t[k]);
       }
+      double* out = hpsi + IDX(p, 0, L);
+      for (int k = 0; k < L; ++k) out[k] = out[k] + t[k];
     }
   }
 }

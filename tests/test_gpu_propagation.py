@@ -136,7 +136,8 @@ def test_propagator_steps(gpu, driver):
     weights = [np.full(len(c), 2.0) for c in chunks]
     hm = api.HamiltonianDevice.from_system(s)
     hm.set_dipole_field([0.0, 0.0, 0.0], positions=x)
-    batches = [batch_of(gpu, h, len(c)) for h, c in zip(hosts, chunks)]
+    batches = [api.Batch(len(c), h.shape[0], True, 8, ff=gpu.from_numpy(h.copy()).cuda())
+               for h, c in zip(hosts, chunks)]
     rho = gpu.full((s.np,), 7.0, dtype=gpu.float64, device="cuda")  # zeroed by the driver
     oh = O.Hamiltonian(s.op, s.v.copy(), s.proj, xyz=x, field=[0.0, 0.0, 0.0])
     te = api.Exponential(4)
