@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -71,27 +72,46 @@ def hamiltonian_bytes(s, nst: int, complex_: bool = True) -> int:
     return b
 
 
+def host_cpu() -> dict:
+    """Model, sockets and cores of this host (lscpu)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "cpus_online": info.get("CPU(s)")}
+
+
 def cpu_baseline_hamiltonian(s, nst: int, budget_s: float) -> dict:
+    """The oracle's H-apply (its C/OpenMP restatement of the reference CPU path, lanes
+    innermost as the reference leaf vectorises them) in a timing build for this host:
+    median of >= 10 applies after 2 warm-ups, bounded by budget_s."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
     import oracle as O
     from octhip import workloads
+    flags = O.use_timing_build()
     z = workloads.gaussian_envelope_states(s, nst)
     psi = workloads.pack(z, s.np_part)
     h = O.Hamiltonian(s.op, s.v, s.proj)
-    h.apply(psi)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
+    for _ in range(2):
         h.apply(psi)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 50:
-            break
-    per = el / reps
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": s.np * nst / per, "unit": "state-points/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} H-applies of one {nst}-state complex batch on the full R=12 sphere "
-                      f"({per * 1e3:.1f} ms each, OpenMP oracle)"}
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 10 or (time.perf_counter() - t_start < budget_s and len(times) < 50):
+        t0 = time.perf_counter()
+        h.apply(psi)
+        times.append(time.perf_counter() - t0)
+    per = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": s.np * nst / per, "unit": "state-points/s", "cores": threads, "kind": "port",
+            "sample": f"median of {len(times)} H-applies (after 2 warm-ups) of one {nst}-state complex batch on the "
+                      f"full R=12 sphere: {per * 1e3:.1f} ms each, OpenMP oracle on {threads} threads",
+            "build": flags, "host": host_cpu()}
 
 
 def bench_hamiltonian(args, torch, api, _lib, workloads):
@@ -446,6 +466,8 @@ def main() -> int:
     ap.add_argument("--streams", type=int, default=1, help="H-apply: batches spread over this many streams")
     ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="H-apply workload only (skip the batched-vs-plain and adjoint parts of the metric)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lattice", action="store_true", help="index neighbours through ri, not the site map")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -476,6 +498,20 @@ def main() -> int:
     fn = {"hamiltonian": bench_hamiltonian, "laplacian": bench_laplacian, "taylor": bench_taylor,
           "adjoint": bench_adjoint, "batched": bench_batched}[args.workload]
     res = fn(args, torch, api, _lib, workloads)
+    if args.workload == "hamiltonian" and not args.no_extras:
+        # the other two parts of the metric, timed in the same run
+        log("batched vs plain launches: 8 uneven spheres, 1 H-apply + 50 Taylor-4 steps")
+        b = bench_batched(args, torch, api, _lib, workloads)
+        res["batched_vs_plain"] = {"value": b["value"], "unit": b["unit"], "plain_s": b["plain_s"],
+                                   "batched_s": b["batched_s"], "workload": b["config"]["workload"],
+                                   "packs": b["config"]["packs"]}
+        log("adjoint: 100 Taylor-4 steps on the R=10 h=0.3 sphere, 32 states, checkpoints every 10")
+        a = bench_adjoint(args, torch, api, _lib, workloads)
+        res["adjoint_over_forward"] = {"value": a["backward_over_forward_step"],
+                                       "unit": "adjoint backward time / plain forward time",
+                                       "adjoint_step_ms": a["value"], "backward_ms": a["backward_ms"],
+                                       "forward_with_checkpoints_ms": a["forward_ms"],
+                                       "plain_forward_ms": a["plain_forward_ms"], "workload": a["config"]["workload"]}
     print(json.dumps(res), flush=True)
     return 0
 

@@ -162,6 +162,40 @@ int octhip_hamiltonian_set_potential(octhip_hamiltonian_t hm, const double* vloc
 int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t hm, const double* positions,
                                         const double* field, void* stream);
 
+/* Periodic systems and k-points (the inputs hamiltonian_elec_update builds,
+ * src/hamiltonian/hamiltonian_elec.F90:1178-1300):
+ * set_periodic: bnd_src = np_part - np ints, the 0-based inner row whose value boundary
+ *   row np+b takes (mesh_periodic_point; the periodic branch of boundaries_set,
+ *   src/grid/boundaries_inc.F90:366-743); NULL returns to zero boundaries.
+ * set_phases: the Bloch phases of one k-point as complex (re, im) pairs:
+ *   phase = hm_base%phase(:, ik), np_part values exp(-i k.x); phase_corr =
+ *   hm_base%phase_corr(:, ik), np_part - np values; projector_phases =
+ *   hm_base%projector_phases(:, 1, imat, ik) for every matrix in order (total_points
+ *   values, as buff_projector_phases).  NULL phase removes them.
+ * With either set, octhip_hamiltonian_apply_batch applies the reference's periodic path
+ * to a batch WITHOUT the phase (hamiltonian_elec_inc.F90:116-154, 243-247): boundary rows
+ * from their images, epsi = phase psi (np_part rows, a work buffer), H on epsi with the
+ * projector phases, hpsi = conj(phase) H epsi.  The propagators (Taylor, Lanczos,
+ * Chebyshev) take batches WITH the phase, as the reference calls them between
+ * set_phase_corr and unset_phase_corr (src/td/propagation_ops_elec.F90:238-246); the
+ * exp-mid/ETRS drivers take batches without it and set/unset it around each batch.
+ * Taylor graphs, batched launches and the adjoint cover finite systems only
+ * (OCTHIP_ERR_UNSUPPORTED on periodic handles).  Synchronise the device first. */
+int octhip_hamiltonian_set_periodic(octhip_hamiltonian_t hm, const int* bnd_src);
+int octhip_hamiltonian_set_phases(octhip_hamiltonian_t hm, const double* phase,
+                                  const double* phase_corr, const double* projector_phases);
+/* X(hamiltonian_elec_apply_batch) on a complex batch whose phase state is given:
+ * psi_has_phase != 0 is psib%has_phase (boundary rows set with phase_corr, no phase
+ * multiplications); 0 is octhip_hamiltonian_apply_batch. */
+int octhip_hamiltonian_apply_batch_phase(octhip_hamiltonian_t hm, double* psi, double* hpsi, int ldf,
+                                         int terms, int set_bc, int psi_has_phase, void* stream);
+/* hamiltonian_elec_base_phase (src/hamiltonian/hamiltonian_elec_base.F90:979-1139):
+ * psi(:, p) = phase(p) src(p) (conjugate: conj(phase(p)) src(p)) for p < np (np <=
+ * np_part); src NULL = in place.  With np = mesh np this is set_phase_corr /
+ * unset_phase_corr (933-975). */
+int octhip_hamiltonian_base_phase(octhip_hamiltonian_t hm, double* psi, const double* src, int ldf,
+                                  int np, int conjugate, void* stream);
+
 /* hpsi = H psi on a packed batch: X(hamiltonian_elec_apply_batch)(hm, mesh, psib, hpsib,
  * terms, set_bc) (src/hamiltonian/hamiltonian_elec_inc.F90:77-261) for the terms
  * kinetic (-1/2m Laplacian), local potential and KB nonlocal projectors, in Octopus's

@@ -209,6 +209,7 @@ extern "C" int octhip_adjoint_create(octhip_hamiltonian_t h, const double* posit
   OCTHIP_REQUIRE(order >= 1 && order <= ADJ_MAX_ORDER, "order %d outside 1..%d", order, ADJ_MAX_ORDER);
   OCTHIP_REQUIRE(checkpoint_every >= 1, "checkpoint_every = %d", checkpoint_every);
   OCTHIP_REQUIRE(vol_pp > 0.0, "vol_pp must be positive");
+  if (is_periodic(h)) return fail(OCTHIP_ERR_UNSUPPORTED, "adjoint: finite systems only (zero boundary, no phases)");
   const int np = h->np;
   auto* a = new octhip_adjoint_s();
   a->h = h;

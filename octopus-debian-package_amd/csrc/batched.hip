@@ -256,6 +256,8 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
   bool any_nl = false, all_nl = true;
   for (int i = 0; i < npairs; ++i) {
     OCTHIP_REQUIRE(hms[i] && psi[i], "pair %d: null handle or batch", i);
+    if (is_periodic(hms[i]))
+      return fail(OCTHIP_ERR_UNSUPPORTED, "pair %d: the batched launch covers finite systems (zero boundary)", i);
     OCTHIP_REQUIRE(ldf[i] >= 1 && ldf[i] <= MAX_LDF, "pair %d: ldf %d outside 1..%d (complex batches)", i, ldf[i],
                    MAX_LDF);
     const octhip_nl_operator_s* op = hms[i]->kin;

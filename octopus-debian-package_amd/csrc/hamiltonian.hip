@@ -27,7 +27,7 @@ void detach_graph(octhip_taylor_graph_t g);
 octhip_hamiltonian_s::WorkSet& workset(octhip_hamiltonian_s* h, hipStream_t s) {
   for (auto& w : h->sets)
     if (w.s == s) return w;
-  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0});
+  h->sets.push_back({s, -1, -1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, -1});
   return h->sets.back();
 }
 
@@ -101,6 +101,7 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s, doub
   b.maps = h->d_maps;
   b.partial = h->d_partial;
   b.pjmax = h->max_nprojs;
+  b.pphase = h->has_kpoint ? h->d_pphase : nullptr;
   int rc = mfma ? launch_bra_mfma(b, h->d_pt, h->pjs, h->nchunk, s, zero_bc, (size_t)np * L,
                                   (size_t)(np_part - np) * L)
                 : launch_bra(b, h->nchunk, false, s);
@@ -138,6 +139,7 @@ int run_bra(octhip_hamiltonian_s* h, const double* f, int L, hipStream_t s, doub
   k.multi_g = h->d_multi_g;
   k.nmulti = h->nmulti;
   k.total_points = h->total_points;
+  k.pphase = h->has_kpoint ? h->d_pphase : nullptr;
   return launch_ket(k, h->nchunk, h->use_mfma, s);
 }
 
@@ -210,6 +212,74 @@ void fill_terms(const octhip_hamiltonian_s* h, int terms, ApplyArgs& a) {
 }
 
 }  // namespace
+
+namespace {
+// the phased copy of a k-point apply's input (np_part rows) of stream s
+int ensure_ephase(octhip_hamiltonian_s* h, int ldf, hipStream_t s, double** out) {
+  auto& w = workset(h, s);
+  if (w.ephase_ldf < ldf) {
+    if (w.ephase) h->retired.push_back(w.ephase);
+    w.ephase = nullptr;
+    w.ephase_ldf = -1;
+    int rc = dalloc((void**)&w.ephase, sizeof(double) * ((size_t)h->kin->np_part << ldf));
+    if (rc) return rc;
+    w.ephase_ldf = ldf;
+  }
+  *out = w.ephase;
+  return OCTHIP_SUCCESS;
+}
+
+// X(hamiltonian_elec_apply_batch) for a periodic handle or one with Bloch phases
+// (src/hamiltonian/hamiltonian_elec_inc.F90:116-154, 243-247): boundaries_set fills the
+// boundary rows from their periodic images (times phase_corr when psi carries the phase);
+// a psi without the phase is copied with it (epsib, np_part rows) and H psi is multiplied
+// by the conjugate phase at the end; the projectors use their Bloch phases; the stencil
+// reads the boundary rows as they are.
+int periodic_apply(octhip_hamiltonian_s* h, double* psi, double* hpsi, int ldf, int is_complex, int terms,
+                   int set_bc, bool has_phase, hipStream_t s) {
+  const int L = 1 << ldf;
+  const int np = h->np, np_part = h->kin->np_part;
+  if (h->has_kpoint && !is_complex) return fail(OCTHIP_ERR_INVALID, "Bloch phases need a complex batch");
+  int rc = ensure_work(h, ldf, s);
+  if (rc) return rc;
+  if (set_bc) {
+    rc = h->periodic ? launch_periodic_bc(psi, h->d_bnd_src, np, h->nbnd, L,
+                                          (has_phase && h->has_kpoint) ? h->d_phase_corr : nullptr, s)
+                     : zero_boundary(psi, np, np_part, L, s);
+    if (rc) return rc;
+  }
+  const double* src = psi;
+  const bool set_phase = h->has_kpoint && !has_phase;
+  if (set_phase) {
+    double* e = nullptr;
+    rc = ensure_ephase(h, ldf, s, &e);
+    if (!rc) rc = launch_phase(e, psi, h->d_phase, np_part, L, false, s);
+    if (rc) return rc;
+    src = e;
+  }
+  if ((terms & OCTHIP_TERM_NON_LOCAL_POTENTIAL) && h->nmat > 0) {
+    rc = run_bra(h, src, L, s, nullptr);
+    if (rc) return rc;
+  }
+  ApplyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  fill_terms(h, terms, a);
+  a.fi = src;
+  a.fo = hpsi;
+  a.L = L;
+  a.lim = np_part;  // boundary rows as set above (or by the caller)
+  rc = launch_operator(h->kin, a, EPI_STORE, s);
+  if (!rc && set_phase) rc = launch_phase(hpsi, hpsi, h->d_phase, np, L, true, s);
+  return rc;
+}
+}  // namespace
+
+namespace octhip {
+int propagator_apply(octhip_hamiltonian_s* h, double* psi, double* hpsi, int ldf, hipStream_t s) {
+  if (is_periodic(h)) return periodic_apply(h, psi, hpsi, ldf, 1, OCTHIP_TERM_ALL, 1, true, s);
+  return octhip_hamiltonian_apply_batch(h, psi, hpsi, ldf, 1, OCTHIP_TERM_ALL, 1, s);
+}
+}  // namespace octhip
 
 extern "C" int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double mass, const double* vlocal,
                                          int nmat, const int* offsets, const double* matrices,
@@ -348,9 +418,13 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_mat_chunk0);
   for (auto& w : h->sets)
     for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2,
-                    (void*)w.dpart, (void*)w.dw})
+                    (void*)w.dpart, (void*)w.dw, (void*)w.ephase})
       dfree(p);
   for (void* p : h->retired) dfree(p);
+  dfree(h->d_bnd_src);
+  dfree(h->d_phase);
+  dfree(h->d_phase_corr);
+  dfree(h->d_pphase);
   drop_krylov(h, nullptr, true);
   for (octhip_taylor_graph_t g : h->graphs) detach_graph(g);
   delete h;
@@ -416,6 +490,7 @@ extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* ps
   OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);
   OCTHIP_REQUIRE(!is_complex || ldf >= 1, "complex batch needs ldf >= 1");
   hipStream_t s = (hipStream_t)stream;
+  if (is_periodic(h)) return periodic_apply(h, psi, hpsi, ldf, is_complex, terms, set_bc, false, s);
   const int L = 1 << ldf;
   const octhip_nl_operator_s* op = h->kin;
   int rc = ensure_work(h, ldf, s);
@@ -488,6 +563,7 @@ extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* p
   OCTHIP_REQUIRE(h && psi, "null argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
+  if (is_periodic(h)) return taylor_apply(h, psi, ldf, deltat, order, imag_time, TaylorOptions(), (hipStream_t)stream);
   int rc = prepare_taylor(h, ldf, (hipStream_t)stream);
   if (rc) return rc;
   return taylor_step(h, psi, ldf, deltat, order, imag_time, nullptr, (hipStream_t)stream, nullptr);
@@ -517,6 +593,8 @@ extern "C" int octhip_taylor_graph_create(octhip_hamiltonian_t h, double* const*
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
   OCTHIP_REQUIRE(order >= 1 && order <= 32, "order %d outside 1..32", order);
   *out = nullptr;
+  if (is_periodic(h))
+    return fail(OCTHIP_ERR_UNSUPPORTED, "taylor_graph: periodic handles and Bloch phases run eagerly");
   // Batch b runs on branch b mod nbr of the graph (the capturing stream and up to three
   // more, forked and joined with events), as the reference gives every batch its own
   // stream (src/td/propagation_ops_elec.F90:236): the kernels of different batches
@@ -644,7 +722,7 @@ void drop_workset(octhip_hamiltonian_s* h, hipStream_t s) {
     if (h->sets[i].s != s) continue;
     const auto w = h->sets[i];
     for (void* p : {(void*)w.partial, (void*)w.proj, (void*)w.ksum, (void*)w.y0, (void*)w.y1, (void*)w.y2,
-                    (void*)w.dpart, (void*)w.dw})
+                    (void*)w.dpart, (void*)w.dw, (void*)w.ephase})
       dfree(p);
     h->sets.erase(h->sets.begin() + (long)i);
     // the mirrored pointers may name the freed set: they are rebound by the next call
@@ -849,24 +927,38 @@ int series_generic(octhip_hamiltonian_s* h, double* psi, int ldf, std::complex<d
     if (rc) return rc;
   }
   double* src = inh ? inh : psi;
-  for (int k = 1; k <= order; ++k) {
-    const double* yin = (k == 1) ? src : h->d_y[(k - 2) & 1];
-    double* yout = h->d_y[(k - 1) & 1];
-    // boundaries_set of the chain's input (its H-apply, set_bc default)
-    rc = run_bra(h, yin, L, s, k == 1 ? src : nullptr);
+  // periodic handles: the chain needs np_part rows (its boundary rows are filled from the
+  // periodic images by every apply), taken from the stream's Krylov buffers
+  double* ychain[2] = {h->d_y[0], h->d_y[1]};
+  if (is_periodic(h)) {
+    double* base = nullptr;
+    rc = krylov_vectors(h, s, 2, ldf, &base);
     if (rc) return rc;
-    if (k == 1 && h->nmat == 0) {
-      rc = zero_boundary(src, h->np, h->kin->np_part, L, s);
+    ychain[0] = base;
+    ychain[1] = base + ((size_t)h->kin->np_part << ldf);
+  }
+  for (int k = 1; k <= order; ++k) {
+    double* yin = (k == 1) ? src : ychain[(k - 2) & 1];
+    double* yout = ychain[(k - 1) & 1];
+    if (is_periodic(h)) {
+      rc = propagator_apply(h, yin, yout, ldf, s);
+    } else {
+      // boundaries_set of the chain's input (its H-apply, set_bc default)
+      rc = run_bra(h, yin, L, s, k == 1 ? src : nullptr);
       if (rc) return rc;
+      if (k == 1 && h->nmat == 0) {
+        rc = zero_boundary(src, h->np, h->kin->np_part, L, s);
+        if (rc) return rc;
+      }
+      ApplyArgs a;
+      std::memset(&a, 0, sizeof(a));
+      fill_terms(h, OCTHIP_TERM_ALL, a);
+      a.fi = yin;
+      a.fo = yout;
+      a.L = L;
+      a.lim = h->np;
+      rc = launch_operator(h->kin, a, EPI_STORE, s);
     }
-    ApplyArgs a;
-    std::memset(&a, 0, sizeof(a));
-    fill_terms(h, OCTHIP_TERM_ALL, a);
-    a.fi = yin;
-    a.fo = yout;
-    a.L = L;
-    a.lim = h->np;
-    rc = launch_operator(h->kin, a, EPI_STORE, s);
     if (!rc) rc = launch_zaxpy(psi, yout, count, z[k].real(), z[k].imag(), s);
     if (!rc && psi2) rc = launch_zaxpy(psi2, yout, count, z2[k].real(), z2[k].imag(), s);
     if (rc) return rc;
@@ -948,7 +1040,8 @@ int taylor_apply(octhip_hamiltonian_s* h, double* psi, int ldf, double deltat, i
   double* dpart = nullptr;
   double* dw = nullptr;
   int rc = OCTHIP_SUCCESS;
-  const bool fused = taylor_deferred(order);
+  // periodic handles (psi carries the Bloch phase): plain passes through the periodic apply
+  const bool fused = taylor_deferred(order) && !is_periodic(h);
   const bool fuse_density = o.density && fused && !o.inh;
   int nchunk = 0;
   if (o.density) {
@@ -1027,7 +1120,11 @@ int exp_apply_batches(octhip_hamiltonian_s* h, double* const* psi, int nbatch, i
       o.weights = weights + (size_t)b * pack;
       o.density = density;
     }
-    int rc = taylor_apply(h, psi[b], ldf, dt, order, 0, o, s);
+    // k-points: set_phase_corr / unset_phase_corr around the apply
+    // (src/td/propagation_ops_elec.F90:238-246)
+    int rc = h->has_kpoint ? launch_phase(psi[b], psi[b], h->d_phase, h->np, 1 << ldf, false, s) : 0;
+    if (!rc) rc = taylor_apply(h, psi[b], ldf, dt, order, 0, o, s);
+    if (!rc && h->has_kpoint) rc = launch_phase(psi[b], psi[b], h->d_phase, h->np, 1 << ldf, true, s);
     if (rc) return rc;
   }
   return OCTHIP_SUCCESS;
@@ -1079,4 +1176,71 @@ extern "C" int octhip_propagator_etrs(octhip_hamiltonian_t h, double* const* psi
     if (rc) return rc;
   }
   return exp_apply_batches(h, psi, nbatch, ldf, 0.5 * dt, order, weights, density, s);
+}
+
+// ------------------------------------------------------- periodic / k-points --
+extern "C" int octhip_hamiltonian_set_periodic(octhip_hamiltonian_t h, const int* bnd_src) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  const int np = h->np, nb = h->kin->np_part - h->np;
+  (void)hipDeviceSynchronize();  // launches in flight may read the old map
+  dfree(h->d_bnd_src);
+  h->d_bnd_src = nullptr;
+  h->nbnd = 0;
+  h->periodic = false;
+  if (!bnd_src) return OCTHIP_SUCCESS;
+  for (int b = 0; b < nb; ++b)
+    OCTHIP_REQUIRE(bnd_src[b] >= 0 && bnd_src[b] < np, "boundary row %d: image %d outside 0..np-1", np + b,
+                   bnd_src[b]);
+  int rc = upload(&h->d_bnd_src, bnd_src, (size_t)nb, nullptr);
+  if (rc) return rc;
+  h->nbnd = nb;
+  h->periodic = true;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_set_phases(octhip_hamiltonian_t h, const double* phase, const double* phase_corr,
+                                             const double* projector_phases) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  const int np_part = h->kin->np_part, nb = np_part - h->np;
+  (void)hipDeviceSynchronize();
+  for (double** p : {&h->d_phase, &h->d_phase_corr, &h->d_pphase}) {
+    dfree(*p);
+    *p = nullptr;
+  }
+  h->has_kpoint = false;
+  if (!phase) return OCTHIP_SUCCESS;
+  OCTHIP_REQUIRE(nb == 0 || phase_corr, "phase_corr is required with the phase (one per boundary row)");
+  OCTHIP_REQUIRE(h->nmat == 0 || projector_phases, "projector_phases are required with projectors");
+  int rc = upload(&h->d_phase, phase, 2 * (size_t)np_part, nullptr);
+  if (!rc && nb > 0) rc = upload(&h->d_phase_corr, phase_corr, 2 * (size_t)nb, nullptr);
+  if (!rc && h->nmat > 0) rc = upload(&h->d_pphase, projector_phases, 2 * (size_t)h->total_points, nullptr);
+  if (rc) return rc;
+  h->has_kpoint = true;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_apply_batch_phase(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
+                                                    int terms, int set_bc, int psi_has_phase, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  OCTHIP_REQUIRE(psi && hpsi && psi != hpsi, "bad batch pointers");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "a complex batch is needed (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(!psi_has_phase || h->has_kpoint, "psi_has_phase without Bloch phases on the handle");
+  OCTHIP_REQUIRE(set_bc || !psi_has_phase || !h->periodic,
+                 "a batch with the phase needs set_bc (the phase correction is applied while setting the boundary)");
+  hipStream_t s = (hipStream_t)stream;
+  if (!is_periodic(h)) return octhip_hamiltonian_apply_batch(h, psi, hpsi, ldf, 1, terms, set_bc, stream);
+  return periodic_apply(h, psi, hpsi, ldf, 1, terms, set_bc, psi_has_phase != 0, s);
+}
+
+extern "C" int octhip_hamiltonian_base_phase(octhip_hamiltonian_t h, double* psi, const double* src, int ldf, int np,
+                                             int conjugate, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h && psi, "null argument");
+  OCTHIP_REQUIRE(h->has_kpoint, "no Bloch phases on this handle");
+  OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "a complex batch is needed (ldf in 1..%d)", MAX_LDF);
+  OCTHIP_REQUIRE(np >= 0 && np <= h->kin->np_part, "np=%d outside 0..np_part", np);
+  return launch_phase(psi, src ? src : psi, h->d_phase, np, 1 << ldf, conjugate != 0, (hipStream_t)stream);
 }

@@ -322,6 +322,10 @@ struct BraArgs {
   const int* __restrict__ maps;       // 1-based
   double* __restrict__ partial;       // (chunk, pjmax, L)
   int pjmax;
+  // Bloch phases of the sphere points (complex pairs, indexed like the maps; complex rows
+  // only): the gathered values are multiplied by them (hamiltonian_elec_base_inc.F90:
+  // 683-692); null: none
+  const double* __restrict__ pphase = nullptr;
 };
 
 struct BraReduceArgs {
@@ -361,6 +365,9 @@ struct KetArgs {
   const int* __restrict__ mat_chunk0 = nullptr;  // nmat+1
   const double* __restrict__ scals = nullptr;
   int pjmax = 0;
+  // Bloch phases of the sphere points: the ket row of point q is multiplied by
+  // conj(pphase[q]) (hamiltonian_elec_base_inc.F90:955-966); null: none
+  const double* __restrict__ pphase = nullptr;
 };
 
 // largest projector count per matrix for which the ket forms its projections itself

@@ -82,7 +82,12 @@ __device__ __forceinline__ void bra_plain_body(const BraArgs& b, int c) {
       for (int jj = 0; jj < BRA_PJB; ++jj) acc[jj] = 0.0;
       for (int q = ch.y + sl; q < ch.z; q += nsl) {
         const int row = b.maps[mapoff + q] - 1;
-        const double x = b.psi[(size_t)row * b.L + l];
+        double x = b.psi[(size_t)row * b.L + l];
+        if (b.pphase) {
+          const double xq = b.psi[(size_t)row * b.L + (l ^ 1)];
+          x = zmul_lane(b.pphase[2 * (size_t)(mapoff + q)], b.pphase[2 * (size_t)(mapoff + q) + 1], x, xq,
+                        (l & 1) != 0);
+        }
 #pragma unroll
         for (int jj = 0; jj < BRA_PJB; ++jj)
           if (jb + jj < npj) acc[jj] = fma(b.mats[moff + (size_t)(jb + jj) * npt + q], x, acc[jj]);
@@ -145,6 +150,17 @@ __device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int l
       }
 #pragma unroll
       for (int k = 0; k < KB; ++k) bv[k] = b.psi[(size_t)row[k] * b.L + col];
+      if (b.pphase) {
+        // Bloch phase of each sphere point; the other lane of the (re, im) pair is the
+        // neighbouring thread's (lanes ii and ii ^ 1 hold the same point)
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          const int qk = q0 + 16 * k + kk;
+          const size_t pi2 = 2 * (size_t)(mapoff + (qk < ch.z ? qk : ch.y));
+          const double xq = __shfl_xor(bv[k], 1);
+          bv[k] = zmul_lane(b.pphase[pi2], b.pphase[pi2 + 1], bv[k], xq, (col & 1) != 0);
+        }
+      }
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const bool ok = q0 + 16 * k + kk < ch.z;
@@ -281,6 +297,16 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
       const double bb = k.partial ? sproj[jj * 16 + ii] : k.proj[(size_t)(pb + jj) * k.L + col];
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((q < ch.z && jok) ? a : 0.0, jok ? bb : 0.0, acc, 0, 0, 0);
     }
+    if (k.pphase) {
+      // ket row of point qo times conj(its Bloch phase); lanes ii ^ 1 hold the pair
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qo = pt0 + 4 * r + kk;
+        const size_t pi2 = 2 * (size_t)(mapoff + (qo < ch.z ? qo : ch.y));
+        const double xq = __shfl_xor(acc[r], 1);
+        acc[r] = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc[r], xq, (col & 1) != 0);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int qo = pt0 + 4 * r + kk;
@@ -306,6 +332,12 @@ __device__ __forceinline__ void ket_plain_body(const KetArgs& k, int c) {
     const double* prow = k.pt + (size_t)(mapoff + q) * k.pjs;
     double acc = 0.0;
     for (int jj = 0; jj < npj; ++jj) acc = fma(prow[jj], k.proj[(size_t)(pb + jj) * k.L + l], acc);
+    if (k.pphase) {
+      double accq = 0.0;  // the other lane of the (re, im) pair
+      for (int jj = 0; jj < npj; ++jj) accq = fma(prow[jj], k.proj[(size_t)(pb + jj) * k.L + (l ^ 1)], accq);
+      const size_t pi2 = 2 * (size_t)(mapoff + q);
+      acc = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc, accq, (l & 1) != 0);
+    }
     k.kbuf[(size_t)(mapoff + q) * k.L + l] = acc;
   }
 }

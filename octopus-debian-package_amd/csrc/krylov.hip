@@ -196,6 +196,14 @@ int check_exp_args(octhip_hamiltonian_t h, const double* psi, int ldf, int order
 }  // namespace
 
 namespace octhip {
+int krylov_vectors(octhip_hamiltonian_s* h, hipStream_t s, int nvec, int ldf, double** base) {
+  KrylovBufs kb;
+  int rc = krylov_bufs(h, s, nvec, ldf, &kb);
+  if (rc) return rc;
+  *base = kb.vec;
+  return OCTHIP_SUCCESS;
+}
+
 void drop_krylov(octhip_hamiltonian_s* h, hipStream_t s, bool all) {
   for (size_t i = 0; i < h->krylov.size();) {
     if (all || h->krylov[i].s == s) {
@@ -252,7 +260,7 @@ extern "C" int octhip_exponential_lanczos_batch(octhip_hamiltonian_t h, double* 
   std::vector<double> res(nst, 0.0);
   int iter = 1;
   for (; iter <= order; ++iter) {
-    rc = octhip_hamiltonian_apply_batch(h, V(iter - 1), V(iter), ldf, 1, OCTHIP_TERM_ALL, 1, stream);
+    rc = propagator_apply(h, V(iter - 1), V(iter), ldf, s);
     if (rc) return rc;
     // classical Gram-Schmidt against vb(l..iter), l = max(1, iter - 1) (Hermitian H):
     // every overlap first, then the subtractions (mesh_batch_inc.F90:1218-1231)
@@ -355,7 +363,7 @@ extern "C" int octhip_exponential_cheby_batch(octhip_hamiltonian_t h, double* ps
     b2 = b1;
     b1 = b0;
     b0 = t;
-    rc = octhip_hamiltonian_apply_batch(h, b1, b0, ldf, 1, OCTHIP_TERM_ALL, 1, stream);
+    rc = propagator_apply(h, b1, b0, ldf, s);
     if (rc) return rc;
     static const cplx mi_pow[4] = {cplx(1.0, 0.0), cplx(0.0, -1.0), cplx(-1.0, 0.0), cplx(0.0, 1.0)};
     const cplx two_pow = cplx(2.0 * mi_pow[j & 3].real(), 2.0 * mi_pow[j & 3].imag());

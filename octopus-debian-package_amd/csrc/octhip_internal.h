@@ -183,6 +183,9 @@ struct octhip_hamiltonian_s {
     // state weights (created on first use)
     double *dpart, *dw;
     int dchunks;
+    // the phased copy of the input of a k-point apply (np_part rows)
+    double* ephase;
+    int ephase_ldf;
   };
   std::vector<WorkSet> sets;
   long graph_sets = 0;
@@ -206,6 +209,16 @@ struct octhip_hamiltonian_s {
     double* small;
   };
   std::vector<KrylovSet> krylov;
+  // periodic boundaries: boundary row np+b takes inner row bnd_src[b]
+  int* d_bnd_src = nullptr;
+  int nbnd = 0;
+  bool periodic = false;
+  // Bloch phases of one k-point (complex pairs): phase (np_part), phase_corr (np_part-np),
+  // projector phases (total_points)
+  double* d_phase = nullptr;
+  double* d_phase_corr = nullptr;
+  double* d_pphase = nullptr;
+  bool has_kpoint = false;
 };
 
 namespace octhip {
@@ -231,6 +244,17 @@ int capture_graph(hipStream_t s, const std::function<int(hipStream_t)>& prepare,
 hipStream_t pin_workset(octhip_hamiltonian_s* h, hipStream_t s);
 // free the work-buffer set kept under key s (no synchronisation: nothing may use it)
 void drop_workset(octhip_hamiltonian_s* h, hipStream_t s);
+// H psi for the propagators: on a handle with periodic boundaries or Bloch phases, psi
+// carries the phase (as between set_phase_corr and unset_phase_corr in the reference's
+// propagation) and its boundary rows are filled from their periodic images; on any other
+// handle this is octhip_hamiltonian_apply_batch with the zero boundary.  psi: np_part rows.
+int propagator_apply(octhip_hamiltonian_s* h, double* psi, double* hpsi, int ldf, hipStream_t s);
+inline bool is_periodic(const octhip_hamiltonian_s* h) { return h->periodic || h->has_kpoint; }
+int launch_periodic_bc(double* f, const int* src, int np, int nb, int L, const double* corr, hipStream_t s);
+int launch_phase(double* dst, const double* src, const double* phase, int n, int L, bool conjugate, hipStream_t s);
+// nvec Krylov-style buffers of np_part rows of the stream's set (Lanczos, Chebyshev, and
+// the Taylor chain of periodic handles)
+int krylov_vectors(octhip_hamiltonian_s* h, hipStream_t s, int nvec, int ldf, double** base);
 // free the Lanczos/Chebyshev buffers of stream s (every stream when all)
 void drop_krylov(octhip_hamiltonian_s* h, hipStream_t s, bool all);
 // field of the dipole term (positions must be uploaded); refreshes v_eff on stream s

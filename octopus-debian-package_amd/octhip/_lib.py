@@ -49,6 +49,10 @@ SIGNATURES = {
     "octhip_hamiltonian_set_potential": [_P, _DP, _P],
     "octhip_hamiltonian_set_dipole_field": [_P, _DP, _DP, _P],
     "octhip_hamiltonian_apply_batch": [_P, _P, _P, _I, _I, _I, _I, _P],
+    "octhip_hamiltonian_set_periodic": [_P, _IP],
+    "octhip_hamiltonian_set_phases": [_P, _DP, _DP, _DP],
+    "octhip_hamiltonian_apply_batch_phase": [_P, _P, _P, _I, _I, _I, _I, _P],
+    "octhip_hamiltonian_base_phase": [_P, _P, _P, _I, _I, _I, _P],
     "octhip_exponential_taylor_batch": [_P, _P, _I, _D, _I, _I, _P],
     "octhip_exponential_apply_batch": [_P, _P, _I, _D, _I, _I, _P, _D, _P, _DP, _P, _P],
     "octhip_density_accumulate": [_P, _I, _I, _I, _P, _P, _P],

@@ -220,6 +220,29 @@ class HamiltonianDevice:
         call("octhip_hamiltonian_set_dipole_field", self.handle, as_ptr(x, C.c_double),
              as_ptr(f, C.c_double), _stream())
 
+    def set_periodic(self, bnd_src) -> None:
+        """Periodic boundaries: boundary row np+b takes inner row bnd_src[b] (None: zero
+        boundaries again)."""
+        b = None if bnd_src is None else np.ascontiguousarray(bnd_src, dtype=np.int32)
+        _torch().cuda.synchronize()
+        call("octhip_hamiltonian_set_periodic", self.handle, as_ptr(b, C.c_int))
+
+    def set_phases(self, phase, phase_corr=None, projector_phases=None) -> None:
+        """Bloch phases of one k-point (complex arrays): hm_base%phase (np_part),
+        phase_corr (np_part - np) and the projector phases (total_points); None removes them."""
+        def cp(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, np.complex128)).view(np.float64)
+        ph, pc, pp = cp(phase), cp(phase_corr), cp(projector_phases)
+        _torch().cuda.synchronize()
+        call("octhip_hamiltonian_set_phases", self.handle, as_ptr(ph, C.c_double), as_ptr(pc, C.c_double),
+             as_ptr(pp, C.c_double))
+
+    def base_phase(self, psib: "Batch", np_rows: int, conjugate: bool, src: "Batch | None" = None) -> None:
+        """hamiltonian_elec_base_phase (src/hamiltonian/hamiltonian_elec_base.F90:979-1139);
+        np_rows = self.np is set_phase_corr (conjugate False) / unset_phase_corr (True)."""
+        call("octhip_hamiltonian_base_phase", self.handle, psib.ptr(), None if src is None else src.ptr(),
+             psib.ldf, int(np_rows), int(bool(conjugate)), _stream())
+
     def release_stream(self) -> None:
         """Free the work buffers this Hamiltonian keeps for the current stream."""
         call("octhip_hamiltonian_release_stream", self.handle, _stream())
@@ -234,8 +257,13 @@ class HamiltonianDevice:
 
 
 def hamiltonian_elec_apply_batch(hm: HamiltonianDevice, psib: Batch, hpsib: Batch, terms: int = TERM_ALL,
-                                 set_bc: bool = True) -> None:
+                                 set_bc: bool = True, has_phase: bool = False) -> None:
+    """has_phase: psib%has_phase (the batch carries the Bloch phase of the handle's k-point)."""
     assert psib.lanes == hpsib.lanes and psib.is_complex == hpsib.is_complex
+    if has_phase:
+        call("octhip_hamiltonian_apply_batch_phase", hm.handle, psib.ptr(), hpsib.ptr(), psib.ldf, int(terms),
+             int(bool(set_bc)), 1, _stream())
+        return
     call("octhip_hamiltonian_apply_batch", hm.handle, psib.ptr(), hpsib.ptr(), psib.ldf,
          int(psib.is_complex), int(terms), int(bool(set_bc)), _stream())
 

@@ -24,14 +24,28 @@ def build() -> Path:
 
 
 _lib = None
+_path = LIB
+FAST = HERE / "_fast" / "liboracle_fast.so"
+
+
+def use_timing_build() -> str:
+    """Switch to the timing build of the same source (-O3 -march=native, built here for this
+    host by `make fast`; its results equal the parity build's).  Returns the flags used, or
+    the parity build's when the timing build cannot be made."""
+    global _lib, _path
+    r = subprocess.run(["make", "-s", "-C", str(HERE), "fast"], capture_output=True, text=True)
+    if r.returncode == 0 and FAST.exists():
+        _lib, _path = None, FAST
+        return "-O3 -march=native -fopenmp"
+    return "-O3 -march=x86-64-v3 -fopenmp (timing build failed: " + (r.stderr.strip()[-200:] or "?") + ")"
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not LIB.exists():
+        if not _path.exists():
             build()
-        l = C.CDLL(str(LIB))
+        l = C.CDLL(str(_path))
         l.oracle_taylor.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int,
                                     C.c_int]
         l.oracle_taylor_ex.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_double, C.c_int,
@@ -143,7 +157,8 @@ def exp_apply_batches(h: Hamiltonian, psis: list, dt: float, order: int = 4, wei
     for b, psi in enumerate(psis):
         h.taylor(psi, dt, order)
         if rho is not None:
-            density_accumulate(psi, h.np, psi.shape[1] // 2, True, weights[b], rho)
+            w = np.asarray(weights[b], np.float64)
+            density_accumulate(psi, h.np, len(w), True, w, rho)
     return rho
 
 
@@ -376,3 +391,53 @@ def cheby_batch(h: "Hamiltonian", psi: np.ndarray, deltat: float, order: int, mi
         z0 += -1.0 * psi2[:rows].view(np.complex128)
     out = psi0[:rows].view(np.complex128) - psi2[:rows].view(np.complex128)
     psi[:rows] = (0.5 * np.exp(-1j * mid * deltat) * out).view(np.float64)
+
+
+# ------------------------------------------------------- periodic / k-points --
+def periodic_apply(h: "Hamiltonian", psi: np.ndarray, bnd_src: np.ndarray, proj=None, phase=None, phase_corr=None,
+                   pphase=None, has_phase: bool = False) -> np.ndarray:
+    """X(hamiltonian_elec_apply_batch) on a periodic cell, optionally with the Bloch phases of
+    one k-point (src/hamiltonian/hamiltonian_elec_inc.F90:116-154, 243-247).  h: the oracle
+    Hamiltonian of the kinetic and local terms (no projectors); proj: the projector set,
+    applied here with its Bloch phases (hamiltonian_elec_base_inc.F90:683-692 bra,
+    955-966 ket).  psi: packed complex (np_part, L), boundary rows set in place
+    (boundaries_set: periodic images, times phase_corr when psi carries the phase).
+    Returns H psi (np, L)."""
+    npt = h.np
+    z = psi.view(np.complex128)
+    if has_phase:
+        z[npt:] = phase_corr[:, None] * z[bnd_src]
+    else:
+        z[npt:] = z[bnd_src]
+    set_phase = phase is not None and not has_phase
+    e = (phase[:, None] * z) if set_phase else z.copy()
+    ep = np.ascontiguousarray(e).view(np.float64)
+    out = h.apply(ep.copy(), TERM_KINETIC | TERM_LOCAL, set_bc=False).view(np.complex128)
+    if proj is not None and proj.nmat > 0:
+        off = 0
+        for a in range(proj.nmat):
+            P, mp, sc = proj.matrix(a)
+            idx = mp - 1
+            ph = pphase[off: off + len(idx)] if pphase is not None else np.ones(len(idx))
+            lpsi = e[idx] * ph[:, None]
+            pr = sc[:, None] * (P.T @ lpsi)
+            out[idx] += (P @ pr) * np.conj(ph)[:, None]
+            off += len(idx)
+    if set_phase:
+        out = np.conj(phase[:npt])[:, None] * out
+    return np.ascontiguousarray(out).view(np.float64)
+
+
+def periodic_taylor(apply, psi: np.ndarray, rows: int, deltat: float, order: int = 4) -> None:
+    """exponential_taylor_series_batch (src/electrons/exponential.F90:677-757) with a given
+    H-apply (apply(psi_np_part) -> H psi (rows, L)), in place on the inner rows."""
+    L = psi.shape[1]
+    z = 1.0 + 0j
+    psi1 = np.zeros_like(psi)
+    for k in range(1, order + 1):
+        z = z * (-1j * deltat) / k
+        hp = apply(psi if k == 1 else psi1)
+        zp = psi[:rows].view(np.complex128)
+        zp += z * hp.view(np.complex128)
+        if k != order:
+            psi1[:rows] = hp
