@@ -254,6 +254,16 @@ int octhip_density_accumulate(const double* psi, int ldf, int np, int is_complex
  * reduction scratch on every call and synchronises the stream. */
 int octhip_mesh_batch_dotp_vector(const double* a, const double* b, int ldf, int np, int is_complex,
                                   double vol, double* dot, void* stream);
+/* octhip_mesh_batch_dotp_vector for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): no counterpart in the
+ * reference.  Same kernels and partition, so the same bits; it allocates nothing and
+ * does not synchronise: the reduction scratch is a fixed device array of the library
+ * (8 MiB + 16 KiB per device, held until the process ends, enough for every row length
+ * up to 2^10 doubles), and the results go to `dot`, a DEVICE array of 2 doubles per
+ * column ((re, im); im = 0 for real rows).  Every call on one device shares that array:
+ * calls that overlap on one GPU (different streams, host threads) race. */
+int octhip_mesh_batch_dotp_vector_noalloc(const double* a, const double* b, int ldf, int np,
+                                          int is_complex, double vol, double* dot, void* stream);
 /* mesh_batch_nrm2 (src/grid/mesh_batch.F90:74-95): nrm2 = sqrt(<a_c|a_c>) per column (host) */
 int octhip_mesh_batch_nrm2(const double* a, int ldf, int np, int is_complex, double vol,
                            double* nrm2, void* stream);

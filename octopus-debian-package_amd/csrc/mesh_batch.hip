@@ -258,3 +258,52 @@ extern "C" int octhip_mesh_batch_dotp_vector(const double* a, const double* b, i
   return OCTHIP_SUCCESS;
 }
 
+
+// ------------------------------------------------------------ no allocation --
+// octhip_mesh_batch_dotp_vector without any allocation: the partials and the results go
+// to a fixed device array of this library instead of a per-call buffer.  Its capacity is
+// the largest row the library handles (2^MAX_LDF real lanes), 8 MiB + 16 KiB.
+namespace {
+constexpr size_t NOALLOC_DOUBLES = 2 * ((size_t)DOT_BLOCKS * (1 << MAX_LDF) + (1 << MAX_LDF));
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+// device address of `scratch` on each device, looked up once with that device current
+// (in host code the array's name is not a device address)
+constexpr int MAX_DEVICES = 64;
+double* scratch_ptr[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  OCTHIP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(OCTHIP_ERR_UNSUPPORTED, "device %d beyond %d", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return OCTHIP_SUCCESS;
+}
+}  // namespace
+
+extern "C" int octhip_mesh_batch_dotp_vector_noalloc(const double* a, const double* b, int ldf, int np, int is_complex,
+                                                     double vol, double* dot, void* stream) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(a && b && dot, "null argument");
+  OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);
+  OCTHIP_REQUIRE(np >= 0, "np < 0");
+  const int L = 1 << ldf;
+  OCTHIP_REQUIRE(dotp_scratch_bytes(L, is_complex) <= sizeof(double) * NOALLOC_DOUBLES,
+                 "rows of %d doubles need %zu bytes of scratch, more than the %zu available", L,
+                 dotp_scratch_bytes(L, is_complex), sizeof(double) * NOALLOC_DOUBLES);
+  double* sp = nullptr;
+  int rc = scratch_of_current_device(&sp);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // results as (re, im) pairs straight into the caller's device array (im = 0 for real rows)
+  double* res = nullptr;
+  return launch_dotp(a, b, L, np, is_complex, vol, sp, s, &res, dot);
+}

@@ -57,6 +57,7 @@ SIGNATURES = {
     "octhip_exponential_apply_batch": [_P, _P, _I, _D, _I, _I, _P, _D, _P, _DP, _P, _P],
     "octhip_density_accumulate": [_P, _I, _I, _I, _P, _P, _P],
     "octhip_mesh_batch_dotp_vector": [_P, _P, _I, _I, _I, _D, _DP, _P],
+    "octhip_mesh_batch_dotp_vector_noalloc": [_P, _P, _I, _I, _I, _D, _P, _P],
     "octhip_mesh_batch_nrm2": [_P, _I, _I, _I, _D, _DP, _P],
     "octhip_mesh_batch_orthogonalization": [_I, _P, _P, _I, _I, _I, _D, _I, _DP, _DP, _P],
     "octhip_exponential_lanczos_batch": [_P, _P, _I, _I, _D, _I, _I, _D, _P, _D, _IP, _DP, _P],
