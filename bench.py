@@ -370,36 +370,18 @@ def bench_adjoint(args, torch, api, _lib, workloads):
 def bench_batched(args, torch, api, _lib, workloads):
     """8 uneven spheres (R = 6..14, h = 0.25, 8..256 states): one H-apply + 50 Taylor-4
     steps, as one batched launch sequence over all (system, batch) pairs and as plain
-    per-batch launches system by system; launch shapes from the autotuner, pack sizes by
-    the time of a Taylor step; both paths use the same packs."""
+    per-batch launches system by system; block sizes and launch shapes from the library's
+    tuner (octhip_hamiltonian_autotune_pack); both paths use the same blocks."""
     log("building the 8 uneven systems")
     systems = workloads.uneven_systems()
     nsteps, dt = 50, args.dt
     plan = []
     for s, nst in systems:
         hm = api.HamiltonianDevice.from_system(s)
-        # pack size by the time of what the workload runs most: Taylor steps of one scratch
-        # batch (launch shapes from the autotuner) times the number of batches
-        best = None
-        te0 = api.Exponential(4)
-        for pack in (8, 16, 32, 64):
-            if pack > max(8, api.pad_pow2(nst)):
-                continue
-            ldf = int(np.log2(2 * pack))
-            hm.autotune(ldf, reps=3)
-            scratch = api.Batch.from_states(workloads.gaussian_envelope_states(s, pack), s.np_part)
-            te0.apply_batch(hm, scratch, dt)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(3):
-                te0.apply_batch(hm, scratch, dt)
-            torch.cuda.synchronize()
-            us = (time.perf_counter() - t0) * 1e6 / 3
-            cost = -(-nst // pack) * us
-            if best is None or cost < best[0]:
-                best = (cost, pack, ldf, us)
-            del scratch
-        hm.autotune(best[2], reps=3)  # leave the chosen shape configured
+        # block size by the time of what the workload runs most (Taylor steps), chosen by
+        # the library's tuner, which also leaves that row length's launch shape configured
+        pack, us = hm.autotune_pack(nst, reps=3)
+        best = (us, pack, int(np.log2(2 * pack)), us / -(-nst // pack))
         batches = []
         for first in range(0, nst, best[1]):
             n = min(best[1], nst - first)

@@ -436,7 +436,15 @@ int octhip_hamiltonian_set_tuning(octhip_hamiltonian_t hm, int key, int value);
  * the reference (its device path has fixed launch shapes). */
 int octhip_hamiltonian_autotune(octhip_hamiltonian_t hm, int ldf, int reps, int force, void* stream,
                                 double* best_us);
-/* the tuning table as text (one line per shape) */
+/* StatesBlockSize by time (src/states/states_elec.F90:447-475, where it is an input
+ * variable, 32 on GPUs): for block sizes 8..64 (up to pad_pow2(nst)) tunes the launch shape
+ * of that row length and times one Taylor-4 step of a scratch batch; returns the block
+ * size with the smallest time for nst states and its time per step (microseconds, all
+ * ceil(nst/pack) batches) and leaves that row length's shape configured.  Kept in the
+ * tuning table keyed by (np, np_part, matrices, nst).  Synchronises the stream. */
+int octhip_hamiltonian_autotune_pack(octhip_hamiltonian_t hm, int nst, int reps, int force, void* stream,
+                                     int* pack_size, double* step_us);
+/* the tuning table as text (one line per shape and per block-size choice) */
 int octhip_tuning_save(const char* path);
 int octhip_tuning_load(const char* path);
 

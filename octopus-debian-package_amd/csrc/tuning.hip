@@ -1,4 +1,4 @@
-// Autotuner of the H-apply launch shape.
+// Autotuner of the H-apply launch shape and of the states block size.
 //
 // For one Hamiltonian and one row length (ldf) it times every candidate launch shape of
 // the fused stencil pass (z-march lane chunk and lanes per thread, the z segment left to
@@ -34,6 +34,19 @@ struct TuneVal {
 };
 
 std::vector<std::pair<TuneKey, TuneVal>> g_table;
+
+// block size (states per packed batch) per (mesh, projectors, number of states)
+struct PackKey {
+  int np, np_part, nmat, nst;
+  bool operator==(const PackKey& o) const {
+    return np == o.np && np_part == o.np_part && nmat == o.nmat && nst == o.nst;
+  }
+};
+struct PackVal {
+  int pack;
+  double us;  // one Taylor-4 step of all nst states in batches of `pack`
+};
+std::vector<std::pair<PackKey, PackVal>> g_packs;
 
 TuneVal* lookup(const TuneKey& k) {
   for (auto& e : g_table)
@@ -143,6 +156,84 @@ extern "C" int octhip_hamiltonian_autotune(octhip_hamiltonian_t h, int ldf, int 
   return OCTHIP_SUCCESS;
 }
 
+// StatesBlockSize (src/states/states_elec.F90:447-475: 32 on GPUs, at most pad_pow2(nst))
+// chosen by time: for each power-of-two block size from 8 to 64 (up to pad_pow2(nst)),
+// the launch shape is tuned for that row length and one Taylor-4 step of a scratch batch
+// is timed; the cost of nst states is that time times the number of batches.
+extern "C" int octhip_hamiltonian_autotune_pack(octhip_hamiltonian_t h, int nst, int reps, int force, void* stream,
+                                                int* pack_size, double* step_us) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr && pack_size != nullptr, "null argument");
+  OCTHIP_REQUIRE(nst >= 1, "nst %d < 1", nst);
+  OCTHIP_REQUIRE(reps >= 1 && reps <= 1000, "reps %d outside 1..1000", reps);
+  const octhip_nl_operator_s* op = h->kin;
+  const PackKey key{op->np, op->np_part, h->nmat, nst};
+  hipStream_t s = (hipStream_t)stream;
+  if (!force)
+    for (const auto& e : g_packs)
+      if (e.first == key) {
+        *pack_size = e.second.pack;
+        if (step_us) *step_us = e.second.us;
+        int lg = 0;
+        while ((1 << lg) < 2 * e.second.pack) ++lg;
+        return octhip_hamiltonian_autotune(h, lg, reps, 0, stream, nullptr);
+      }
+  int top = 8;
+  while (top < nst && top < 64) top *= 2;
+  const int Lmax = 2 * top;
+  double* psi = nullptr;
+  int rc = dalloc((void**)&psi, sizeof(double) * (size_t)op->np_part * Lmax);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(OCTHIP_ERR_HIP, "autotune_pack: hipEventCreate failed");
+  PackVal best{0, 1e300};
+  int best_ldf = 0;
+  for (int pack = 8; pack <= top && !rc; pack *= 2) {
+    int ldf = 0;
+    while ((1 << ldf) < 2 * pack) ++ldf;
+    const int L = 1 << ldf;
+    rc = octhip_hamiltonian_autotune(h, ldf, reps, 0, stream, nullptr);
+    if (rc) break;
+    hipLaunchKernelGGL(k_fill_pattern, dim3(2048), dim3(256), 0, s, psi, (size_t)op->np_part * L);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      rc = fail(OCTHIP_ERR_HIP, "autotune_pack fill: %s", hipGetErrorString(e));
+      break;
+    }
+    // a short step keeps the scratch values bounded over the repetitions
+    rc = octhip_exponential_taylor_batch(h, psi, ldf, 1e-4, 4, 0, stream);  // warm-up
+    if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "autotune_pack: event record");
+    for (int r = 0; r < reps && !rc; ++r) rc = octhip_exponential_taylor_batch(h, psi, ldf, 1e-4, 4, 0, stream);
+    if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "autotune_pack: event record");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "autotune_pack: event sync");
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(OCTHIP_ERR_HIP, "autotune_pack: elapsed");
+    if (rc) break;
+    const double cost = (double)((nst + pack - 1) / pack) * 1e3 * ms / reps;
+    if (cost < best.us) {
+      best = PackVal{pack, cost};
+      best_ldf = ldf;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  dfree(psi);
+  if (rc) return rc;
+  rc = octhip_hamiltonian_autotune(h, best_ldf, reps, 0, stream, nullptr);  // leave its shape configured
+  if (rc) return rc;
+  bool found = false;
+  for (auto& e : g_packs)
+    if (e.first == key) {
+      e.second = best;
+      found = true;
+    }
+  if (!found) g_packs.push_back({key, best});
+  *pack_size = best.pack;
+  if (step_us) *step_us = best.us;
+  return OCTHIP_SUCCESS;
+}
+
 extern "C" int octhip_tuning_save(const char* path) {
   OCTHIP_REQUIRE(path != nullptr, "null path");
   FILE* f = std::fopen(path, "w");
@@ -152,6 +243,10 @@ extern "C" int octhip_tuning_save(const char* path) {
     std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %.3f\n", e.first.np, e.first.np_part, e.first.nmat, e.first.ldf,
                  e.second.kernel, e.second.lane_chunk, e.second.lpt, e.second.zseg, e.second.gather_tile,
                  e.second.bra, e.second.us);
+  std::fprintf(f, "# pack np np_part nmat nst block_size us_per_step\n");
+  for (const auto& e : g_packs)
+    std::fprintf(f, "pack %d %d %d %d %d %.3f\n", e.first.np, e.first.np_part, e.first.nmat, e.first.nst,
+                 e.second.pack, e.second.us);
   std::fclose(f);
   return OCTHIP_SUCCESS;
 }
@@ -164,6 +259,24 @@ extern "C" int octhip_tuning_load(const char* path) {
   int n = 0;
   while (std::fgets(line, sizeof(line), f)) {
     if (line[0] == '#') continue;
+    if (std::strncmp(line, "pack ", 5) == 0) {
+      PackKey k;
+      PackVal v;
+      if (std::sscanf(line + 5, "%d %d %d %d %d %lf", &k.np, &k.np_part, &k.nmat, &k.nst, &v.pack, &v.us) != 6 ||
+          v.pack < 1 || v.pack > 512 || (v.pack & (v.pack - 1)) != 0) {
+        std::fclose(f);
+        return fail(OCTHIP_ERR_INVALID, "%s: malformed block-size line %d", path, n + 1);
+      }
+      bool found = false;
+      for (auto& e : g_packs)
+        if (e.first == k) {
+          e.second = v;
+          found = true;
+        }
+      if (!found) g_packs.push_back({k, v});
+      ++n;
+      continue;
+    }
     TuneKey k;
     TuneVal v;
     if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %lf", &k.np, &k.np_part, &k.nmat, &k.ldf, &v.kernel,

@@ -77,6 +77,7 @@ SIGNATURES = {
     "octhip_batched_destroy": [_P],
     "octhip_batched_graph_create": [_P, _P, _P],
     "octhip_hamiltonian_autotune": [_P, _I, _I, _I, _P, _P],
+    "octhip_hamiltonian_autotune_pack": [_P, _I, _I, _I, _P, _IP, _DP],
     "octhip_tuning_save": [C.c_char_p],
     "octhip_tuning_load": [C.c_char_p],
     "octhip_profiling_select": [C.c_char_p],

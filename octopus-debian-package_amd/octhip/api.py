@@ -210,6 +210,15 @@ class HamiltonianDevice:
              C.byref(us))
         return us.value
 
+    def autotune_pack(self, nst: int, reps: int = 3, force: bool = False) -> tuple[int, float]:
+        """StatesBlockSize for nst states by time (block sizes 8..64); leaves the launch
+        shape of that row length configured.  Returns (block size, us per Taylor-4 step of
+        all nst states)."""
+        pack, us = C.c_int(), C.c_double()
+        call("octhip_hamiltonian_autotune_pack", self.handle, int(nst), int(reps), int(bool(force)), _stream(),
+             C.byref(pack), C.byref(us))
+        return pack.value, us.value
+
     def set_potential(self, v: np.ndarray) -> None:
         v = np.ascontiguousarray(v, dtype=np.float64)
         call("octhip_hamiltonian_set_potential", self.handle, as_ptr(v, C.c_double), _stream())
