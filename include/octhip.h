@@ -153,6 +153,15 @@ int octhip_hamiltonian_destroy(octhip_hamiltonian_t hm);
  * otherwise stay allocated until destroy.  Synchronises the stream first; buffers
  * captured into a Taylor graph are not affected. */
 int octhip_hamiltonian_release_stream(octhip_hamiltonian_t hm, void* stream);
+/* Device memory held by a Hamiltonian handle, in bytes: bytes[0] its data (potential,
+ * projector tables, periodic tables and phases), bytes[1] the work buffers of the streams
+ * that called it (projector and Taylor buffers, Krylov vectors), bytes[2] the work sets
+ * handed to Taylor graphs (freed with each graph), bytes[3] retired work buffers (outgrown
+ * by a longer row; kept until octhip_hamiltonian_trim or destroy).  No counterpart in the
+ * reference, whose device buffers are sized by the states block size once. */
+int octhip_hamiltonian_memory(octhip_hamiltonian_t hm, size_t bytes[4]);
+/* Free the retired work buffers (synchronises the device first). */
+int octhip_hamiltonian_trim(octhip_hamiltonian_t hm);
 /* replace the local potential (np doubles, host) -- hamiltonian_elec_update */
 int octhip_hamiltonian_set_potential(octhip_hamiltonian_t hm, const double* vlocal, void* stream);
 /* v_eff(p) = vlocal(p) + sum_d field[d] * x_d(p): the length-gauge dipole term of

@@ -482,6 +482,61 @@ extern "C" int octhip_hamiltonian_release_stream(octhip_hamiltonian_t h, void* s
   return OCTHIP_SUCCESS;
 }
 
+namespace {
+size_t dev_bytes(const void* p) {
+  if (!p) return 0;
+  size_t n = 0;
+  if (hipMemPtrGetInfo(const_cast<void*>(p), &n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+}  // namespace
+
+extern "C" int octhip_hamiltonian_memory(octhip_hamiltonian_t h, size_t* bytes) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr && bytes != nullptr, "null argument");
+  size_t data = 0, work = 0, graph = 0, old = 0;
+  for (const void* p : {(const void*)h->d_v, (const void*)h->d_xyz, (const void*)h->d_veff,
+                        (const void*)h->d_matrices, (const void*)h->d_maps, (const void*)h->d_scals,
+                        (const void*)h->d_pt, (const void*)h->d_pos, (const void*)h->d_ent,
+                        (const void*)h->d_mat_pbase, (const void*)h->d_mat_npj, (const void*)h->d_chunks,
+                        (const void*)h->d_mat_chunk0, (const void*)h->d_mat_info, (const void*)h->d_proj_mat,
+                        (const void*)h->d_kslot, (const void*)h->d_multi_pos, (const void*)h->d_multi_g,
+                        (const void*)h->d_bnd_src, (const void*)h->d_phase, (const void*)h->d_phase_corr,
+                        (const void*)h->d_pphase})
+    data += dev_bytes(p);
+  for (const auto& w : h->sets) {
+    size_t b = 0;
+    for (const void* p : {(const void*)w.partial, (const void*)w.proj, (const void*)w.ksum, (const void*)w.y0,
+                          (const void*)w.y1, (const void*)w.y2, (const void*)w.dpart, (const void*)w.dw,
+                          (const void*)w.ephase})
+      b += dev_bytes(p);
+    // sets handed to Taylor graphs are kept under negative keys
+    if (reinterpret_cast<intptr_t>(w.s) < 0) graph += b;
+    else work += b;
+  }
+  for (const auto& k : h->krylov) work += dev_bytes(k.vec) + dev_bytes(k.small);
+  for (const void* p : h->retired) old += dev_bytes(p);
+  bytes[0] = data;
+  bytes[1] = work;
+  bytes[2] = graph;
+  bytes[3] = old;
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_hamiltonian_trim(octhip_hamiltonian_t h) {
+  OCTHIP_REQUIRE_INIT();
+  OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  // retired buffers belong to no set and no graph; work queued before they were retired
+  // may still read them, so wait for the device first
+  OCTHIP_HIP(hipDeviceSynchronize());
+  for (void* p : h->retired) dfree(p);
+  h->retired.clear();
+  return OCTHIP_SUCCESS;
+}
+
 extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
                                               int is_complex, int terms, int set_bc, void* stream) {
   OCTHIP_REQUIRE_INIT();

@@ -252,6 +252,17 @@ class HamiltonianDevice:
         call("octhip_hamiltonian_base_phase", self.handle, psib.ptr(), None if src is None else src.ptr(),
              psib.ldf, int(np_rows), int(bool(conjugate)), _stream())
 
+    def memory(self) -> dict:
+        """Device bytes held by this handle: data, per-stream work buffers, sets held by
+        Taylor graphs, retired buffers (octhip_hamiltonian_memory)."""
+        b = (C.c_size_t * 4)()
+        call("octhip_hamiltonian_memory", self.handle, b)
+        return dict(data=b[0], work=b[1], graphs=b[2], retired=b[3])
+
+    def trim(self) -> None:
+        """Free the retired work buffers (synchronises the device)."""
+        call("octhip_hamiltonian_trim", self.handle)
+
     def release_stream(self) -> None:
         """Free the work buffers this Hamiltonian keeps for the current stream."""
         call("octhip_hamiltonian_release_stream", self.handle, _stream())

@@ -156,3 +156,44 @@ def test_taylor_graph_create_destroy_frees_work_sets(gpu):
     _lib.call("octhip_hamiltonian_destroy", hm_handle)
     del g
     gc.collect()
+
+
+def test_memory_report_and_trim(gpu):
+    """octhip_hamiltonian_memory names what a handle holds: growing the row length
+    retires the smaller work buffers, a graph's sets are reported as the graph's, trim
+    frees the retired ones and the handle still computes the same H psi afterwards."""
+    import gc
+    print("[hamiltonian memory / trim]", flush=True)
+    s = small_sphere(radius=4.0, spacing=0.25, natoms=3, rc=1.2)
+    hm = api.HamiltonianDevice.from_system(s)
+    m0 = hm.memory()
+    assert m0["data"] > 0 and m0["work"] == 0 and m0["graphs"] == 0 and m0["retired"] == 0
+    small = api.Batch.from_states(workloads.gaussian_envelope_states(s, 4, width=2.0), s.np_part)
+    big = api.Batch.from_states(workloads.gaussian_envelope_states(s, 32, width=2.0), s.np_part)
+    out_s, out_b = api.Batch(4, s.np, True), api.Batch(32, s.np, True)
+    api.hamiltonian_elec_apply_batch(hm, small, out_s)
+    api.Exponential(4).apply_batch(hm, small.clone(), 0.01)
+    m1 = hm.memory()
+    assert m1["work"] > 0 and m1["retired"] == 0
+    api.hamiltonian_elec_apply_batch(hm, big, out_b)
+    api.Exponential(4).apply_batch(hm, big.clone(), 0.01)
+    m2 = hm.memory()
+    assert m2["retired"] > 0 and m2["work"] > m1["work"]
+    g = api.TaylorGraph(api.Exponential(4), hm, [big.clone()], 0.01)
+    m3 = hm.memory()
+    assert m3["graphs"] > 0
+    del g
+    gc.collect()
+    gpu.cuda.synchronize()
+    assert hm.memory()["graphs"] == 0
+    ref_b = out_b.ff.clone()
+    hm.trim()
+    m4 = hm.memory()
+    assert m4["retired"] == 0 and m4["work"] == m2["work"] and m4["data"] == m0["data"]
+    api.hamiltonian_elec_apply_batch(hm, big, out_b)
+    api.hamiltonian_elec_apply_batch(hm, small, out_s)
+    gpu.cuda.synchronize()
+    assert gpu.equal(out_b.ff, ref_b)
+    ref = O.Hamiltonian(s.op, s.v, s.proj).apply(workloads.pack(workloads.gaussian_envelope_states(s, 4, width=2.0),
+                                                                s.np_part))
+    assert max_rel_err(out_s.ff.cpu().numpy(), ref) <= 1e-12
