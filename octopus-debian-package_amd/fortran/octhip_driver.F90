@@ -8,7 +8,9 @@
 !   int32 maps; float64 scals; float64 psi(L*np_part); float64 deltat; int32 order.
 ! output: float64 lap(L*np) (factor -1/2), hpsi(L*np), psi after one Taylor step
 !   (L*np_part), that psi unpacked on the device (L/2 complex states of np points, state
-!   after state: batch_read_device_to_unpacked) and packed again into a fresh batch (L*np).
+!   after state: batch_read_device_to_unpacked) and packed again into a fresh batch (L*np),
+!   the mesh inner products <psi_c|psi_c> of that psi with volume element 1 (L doubles: L/2
+!   (re, im) pairs), and the handle's device bytes (4 values: data, work, graphs, retired).
 !   Exit status 0 on success; on a library error the message of octhip_get_last_error is
 !   printed and the status is 2.
 program octhip_driver
@@ -21,6 +23,8 @@ program octhip_driver
     unp(:), repacked(:)
   integer(c_int), allocatable, target :: points(:), ri(:), rimap_inv(:), offsets(:), maps(:)
   real(c_double) :: deltat
+  real(c_double), allocatable :: dots(:)
+  integer(c_size_t) :: mem(4)
   type(c_ptr) :: op, hm, d_psi, d_out, d_unp, s
   character(len=4096) :: fin, fout
   integer :: u
@@ -79,10 +83,14 @@ program octhip_driver
   call check(octhip_batch_pack(d_unp, int(np, c_size_t), L / 2, np, 1_c_int, d_out, ldf, 0_c_int, s), &
     'batch_pack')
   call check(octhip_memcpy_dtoh(c_loc(repacked), d_out, int(8, c_size_t) * L * np, s), 'dtoh repacked')
+  ! mesh_batch_dotp_vector of the propagated batch with itself, and the handle's memory
+  allocate(dots(L))
+  call check(octhip_mesh_batch_dotp_vector(d_psi, d_psi, ldf, np, 1_c_int, 1.0_c_double, dots, s), 'dotp')
+  call check(octhip_hamiltonian_memory(hm, mem), 'memory')
   call check(octhip_stream_synchronize(s), 'synchronize')
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-  write(u) lap, hpsi, psi, unp, repacked
+  write(u) lap, hpsi, psi, unp, repacked, dots, real(mem, c_double)
   close(u)
 
   call check(octhip_hamiltonian_destroy(hm), 'hamiltonian_destroy')

@@ -32,6 +32,15 @@ module octhip_m
     octhip_adjoint_create, octhip_adjoint_forward, octhip_adjoint_backward, octhip_adjoint_destroy, &
     octhip_batch_pack, octhip_batch_unpack, octhip_exponential_apply_batch, octhip_density_accumulate, &
     octhip_propagator_expmid, octhip_propagator_etrs, &
+    octhip_memcpy_dtod, octhip_nl_operator_set_tuning, octhip_nl_operator_get_tuning, &
+    octhip_nl_operator_uses_lattice, octhip_hamiltonian_set_tuning, octhip_hamiltonian_autotune, &
+    octhip_hamiltonian_autotune_pack, octhip_tuning_save, octhip_tuning_load, &
+    octhip_hamiltonian_memory, octhip_hamiltonian_trim, &
+    octhip_hamiltonian_set_periodic, octhip_hamiltonian_set_phases, octhip_hamiltonian_apply_batch_phase, &
+    octhip_hamiltonian_base_phase, octhip_exponential_lanczos_batch, octhip_exponential_cheby_batch, &
+    octhip_mesh_batch_dotp_vector, octhip_mesh_batch_dotp_vector_noalloc, octhip_mesh_batch_nrm2, &
+    octhip_mesh_batch_orthogonalization, octhip_batched_create, octhip_batched_apply, &
+    octhip_batched_taylor_step, octhip_batched_graph_create, octhip_batched_destroy, &
     octhip_last_error_message
 
   interface
@@ -244,6 +253,178 @@ module octhip_m
     integer(c_int) function octhip_adjoint_destroy(adj) bind(C, name="octhip_adjoint_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: adj
+    end function
+
+    integer(c_int) function octhip_memcpy_dtod(dst, src, bytes, stream) bind(C, name="octhip_memcpy_dtod")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: dst, src, stream
+      integer(c_size_t), value :: bytes
+    end function
+
+    ! launch-shape knobs, the autotuners and their persisted table
+    integer(c_int) function octhip_nl_operator_set_tuning(op, key, val) bind(C, name="octhip_nl_operator_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+      integer(c_int), value :: key, val
+    end function
+    integer(c_int) function octhip_nl_operator_get_tuning(op, key, val) bind(C, name="octhip_nl_operator_get_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+      integer(c_int), value :: key
+      integer(c_int), intent(out) :: val
+    end function
+    integer(c_int) function octhip_nl_operator_uses_lattice(op, flag) bind(C, name="octhip_nl_operator_uses_lattice")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+      integer(c_int), intent(out) :: flag
+    end function
+    integer(c_int) function octhip_hamiltonian_set_tuning(hm, key, val) bind(C, name="octhip_hamiltonian_set_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm
+      integer(c_int), value :: key, val
+    end function
+    integer(c_int) function octhip_hamiltonian_autotune(hm, ldf, reps, force, stream, best_us) &
+        bind(C, name="octhip_hamiltonian_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, stream
+      integer(c_int), value :: ldf, reps, force
+      real(c_double), intent(out) :: best_us
+    end function
+    ! StatesBlockSize by time (src/states/states_elec.F90:447-475)
+    integer(c_int) function octhip_hamiltonian_autotune_pack(hm, nst, reps, force, stream, block_size, step_us) &
+        bind(C, name="octhip_hamiltonian_autotune_pack")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, stream
+      integer(c_int), value :: nst, reps, force
+      integer(c_int), intent(out) :: block_size
+      real(c_double), intent(out) :: step_us
+    end function
+    ! path: a NUL-terminated string (trim(name)//c_null_char)
+    integer(c_int) function octhip_tuning_save(path) bind(C, name="octhip_tuning_save")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function octhip_tuning_load(path) bind(C, name="octhip_tuning_load")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    ! device bytes held by a handle: data, stream work sets, graph sets, retired buffers
+    integer(c_int) function octhip_hamiltonian_memory(hm, bytes) bind(C, name="octhip_hamiltonian_memory")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: hm
+      integer(c_size_t), intent(out) :: bytes(4)
+    end function
+    integer(c_int) function octhip_hamiltonian_trim(hm) bind(C, name="octhip_hamiltonian_trim")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm
+    end function
+
+    ! periodic boundaries and the Bloch phases of one k-point (src/grid/boundaries_inc.F90:
+    ! 366-743, src/hamiltonian/hamiltonian_elec_base.F90:979-1139); host arrays or c_null_ptr
+    integer(c_int) function octhip_hamiltonian_set_periodic(hm, bnd_src) bind(C, name="octhip_hamiltonian_set_periodic")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, bnd_src
+    end function
+    integer(c_int) function octhip_hamiltonian_set_phases(hm, phase, phase_corr, projector_phases) &
+        bind(C, name="octhip_hamiltonian_set_phases")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, phase, phase_corr, projector_phases
+    end function
+    integer(c_int) function octhip_hamiltonian_apply_batch_phase(hm, psi, hpsi, ldf, terms, set_bc, has_phase, &
+        stream) bind(C, name="octhip_hamiltonian_apply_batch_phase")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, psi, hpsi, stream
+      integer(c_int), value :: ldf, terms, set_bc, has_phase
+    end function
+    integer(c_int) function octhip_hamiltonian_base_phase(hm, psi, src, ldf, np, conjugate, stream) &
+        bind(C, name="octhip_hamiltonian_base_phase")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hm, psi, src, stream
+      integer(c_int), value :: ldf, np, conjugate
+    end function
+
+    ! exponential_lanczos_batch / exponential_cheby_batch (src/electrons/exponential.F90:766-971)
+    integer(c_int) function octhip_exponential_lanczos_batch(hm, psi, ldf, nst, deltat, imag_time, order, tol, &
+        inh, vol, iterations, residual, stream) bind(C, name="octhip_exponential_lanczos_batch")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, psi, inh, stream
+      integer(c_int), value :: ldf, nst, imag_time, order
+      real(c_double), value :: deltat, tol, vol
+      integer(c_int), intent(out) :: iterations
+      real(c_double), intent(out) :: residual
+    end function
+    integer(c_int) function octhip_exponential_cheby_batch(hm, psi, ldf, deltat, order, mid, half_span, stream) &
+        bind(C, name="octhip_exponential_cheby_batch")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: hm, psi, stream
+      integer(c_int), value :: ldf, order
+      real(c_double), value :: deltat, mid, half_span
+    end function
+
+    ! mesh_batch_dotp_vector, mesh_batch_nrm2, mesh_batch_orthogonalization
+    ! (src/grid/mesh_batch_inc.F90); dot/nrm2/overlap/norm are host arrays
+    integer(c_int) function octhip_mesh_batch_dotp_vector(a, b, ldf, np, is_complex, vol, dot, stream) &
+        bind(C, name="octhip_mesh_batch_dotp_vector")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: a, b, stream
+      integer(c_int), value :: ldf, np, is_complex
+      real(c_double), value :: vol
+      real(c_double), intent(out) :: dot(*)
+    end function
+    ! the same without allocating; dot is a device array of 2 doubles per column
+    integer(c_int) function octhip_mesh_batch_dotp_vector_noalloc(a, b, ldf, np, is_complex, vol, dot, stream) &
+        bind(C, name="octhip_mesh_batch_dotp_vector_noalloc")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: a, b, dot, stream
+      integer(c_int), value :: ldf, np, is_complex
+      real(c_double), value :: vol
+    end function
+    integer(c_int) function octhip_mesh_batch_nrm2(a, ldf, np, is_complex, vol, nrm2, stream) &
+        bind(C, name="octhip_mesh_batch_nrm2")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: a, stream
+      integer(c_int), value :: ldf, np, is_complex
+      real(c_double), value :: vol
+      real(c_double), intent(out) :: nrm2(*)
+    end function
+    integer(c_int) function octhip_mesh_batch_orthogonalization(nvec, psi, phi, ldf, nst, np, vol, normalize, &
+        overlap, norm, stream) bind(C, name="octhip_mesh_batch_orthogonalization")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), intent(in) :: psi(*)
+      type(c_ptr), value :: phi, overlap, norm, stream
+      integer(c_int), value :: nvec, ldf, nst, np, normalize
+      real(c_double), value :: vol
+    end function
+
+    ! several (Hamiltonian, batch) pairs in one launch per kernel kind; psi, hpsi: c_loc of
+    ! arrays of device pointers (hpsi may be c_null_ptr)
+    integer(c_int) function octhip_batched_create(npairs, hms, psi, hpsi, ldf, deltat, order, batched) &
+        bind(C, name="octhip_batched_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: npairs, order
+      type(c_ptr), intent(in) :: hms(*)
+      type(c_ptr), value :: psi, hpsi
+      integer(c_int), intent(in) :: ldf(*)
+      real(c_double), value :: deltat
+      type(c_ptr), intent(out) :: batched
+    end function
+    integer(c_int) function octhip_batched_apply(batched, stream) bind(C, name="octhip_batched_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batched, stream
+    end function
+    integer(c_int) function octhip_batched_taylor_step(batched, stream) bind(C, name="octhip_batched_taylor_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batched, stream
+    end function
+    integer(c_int) function octhip_batched_graph_create(batched, stream, graph) &
+        bind(C, name="octhip_batched_graph_create")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batched, stream
+      type(c_ptr), intent(out) :: graph
+    end function
+    integer(c_int) function octhip_batched_destroy(batched) bind(C, name="octhip_batched_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batched
     end function
   end interface
 

@@ -75,7 +75,14 @@ def test_fortran_driver_on_device(tmp_path):
     psi1 = out[2 * s.np * L: 2 * s.np * L + s.np_part * L].reshape(s.np_part, L)
     rest = out[2 * s.np * L + s.np_part * L:]
     unp = rest[: s.np * L].view(np.complex128).reshape(L // 2, s.np)
-    repacked = rest[s.np * L:].reshape(s.np, L)
+    repacked = rest[s.np * L: 2 * s.np * L].reshape(s.np, L)
+    dots = rest[2 * s.np * L: 2 * s.np * L + L].view(np.complex128)
+    mem = rest[2 * s.np * L + L:]
+    assert mem.shape == (4,) and mem[0] > 0 and mem[1] > 0
+    # mesh_batch_dotp_vector through the Fortran binding: |psi_c|^2 summed over the mesh
+    zz = psi1[: s.np].view(np.complex128)
+    ref_dots = np.sum(np.conj(zz) * zz, axis=0)
+    assert np.max(np.abs(dots - ref_dots)) <= 1e-13 * np.max(np.abs(ref_dots))
     # device unpack / pack: the host packer's layout, bit for bit
     assert np.array_equal(unp, workloads.unpack(psi1, L // 2, s.np, True))
     assert np.array_equal(repacked, psi1[: s.np])
