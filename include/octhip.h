@@ -116,8 +116,12 @@ int octhip_nl_operator_uses_lattice(octhip_nl_operator_t op, int* flag);
  * over j in stencil order with fused multiply-adds, like the C leaf
  * (src/grid/operate_inc.c:76-118).  Boundary rows (np..np_part-1) of fi are READ as
  * they are: the caller sets them (boundaries_set), as Octopus does before operate.
- * Only points == OCTHIP_OP_ALL is supported (inner/outer splits exist only with domain
- * parallelisation). */
+ * points: OCTHIP_OP_ALL, or the inner / outer runs only (op%inner / op%outer of
+ * nl_operator_build, src/grid/nl_operator.F90:457-514: a run is inner when every
+ * neighbour of its points is an inner point), through the run-compressed leaf with the
+ * same per-lane arithmetic; rows of the other part are not written.  The split is built
+ * for every operator, so a domain's operator can overlap its ghost exchange with the
+ * inner points as the reference does. */
 int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const double* fi, double* fo,
                                      int ldf, double factor, int points, void* stream);
 

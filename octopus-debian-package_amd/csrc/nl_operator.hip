@@ -456,6 +456,26 @@ extern "C" int octhip_nl_operator_create(int n, const double* w, const int* sten
   hipStream_t s = nullptr;
   int rc = upload(&op->d_ri, ri, (size_t)nri * n, s);
   if (!rc) rc = upload(&op->d_rimap, rimap.data(), rimap.size(), s);
+  // inner / outer runs: a run is inner when its last point plus the largest offset is
+  // still an inner point (maxp <= np in the reference's 1-based terms)
+  {
+    std::vector<int> pri[2], pmin[2], pmax[2];
+    for (int l = 0; l < nri; ++l) {
+      int mx = ri[(size_t)l * n];
+      for (int j = 1; j < n; ++j) mx = std::max(mx, ri[(size_t)l * n + j]);
+      const int part = (long)rimap_inv[l + 1] + mx <= np ? 0 : 1;
+      pri[part].insert(pri[part].end(), ri + (size_t)l * n, ri + (size_t)(l + 1) * n);
+      pmin[part].push_back(rimap_inv[l]);
+      pmax[part].push_back(rimap_inv[l + 1]);
+    }
+    for (int part = 0; part < 2 && !rc; ++part) {
+      op->part_nri[part] = (int)pmin[part].size();
+      if (pmin[part].empty()) continue;
+      rc = upload(&op->d_part_ri[part], pri[part].data(), pri[part].size(), s);
+      if (!rc) rc = upload(&op->d_part_imin[part], pmin[part].data(), pmin[part].size(), s);
+      if (!rc) rc = upload(&op->d_part_imax[part], pmax[part].data(), pmax[part].size(), s);
+    }
+  }
   if (!rc && stencil_points) {
     bool ok = false;
     if (coords) {
@@ -506,6 +526,11 @@ extern "C" int octhip_nl_operator_destroy(octhip_nl_operator_t op) {
   for (void* p : op->retired) dfree(p);
   dfree(op->d_ri);
   dfree(op->d_rimap);
+  for (int part = 0; part < 2; ++part) {
+    dfree(op->d_part_ri[part]);
+    dfree(op->d_part_imin[part]);
+    dfree(op->d_part_imax[part]);
+  }
   delete op;
   return OCTHIP_SUCCESS;
 }
@@ -585,6 +610,12 @@ void stencil_args(const octhip_nl_operator_s* op, double factor, ApplyArgs& a) {
 }
 }  // namespace octhip
 
+namespace octhip {
+// the run-compressed leaf over nri runs given by their ri rows and point ranges
+int launch_ri_runs(int n, const double* w, int nri, const int* ri, const int* imin, const int* imax,
+                   const double* fi, int ldf, double* fo, hipStream_t s);
+}  // namespace octhip
+
 extern "C" int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const double* fi, double* fo,
                                                 int ldf, double factor, int points, void* stream) {
   OCTHIP_REQUIRE_INIT();
@@ -592,8 +623,17 @@ extern "C" int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const d
   OCTHIP_REQUIRE(fi && fo, "null batch pointer");
   OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);
   OCTHIP_REQUIRE(fi != fo, "operate_batch cannot work in place");
-  if (points != OCTHIP_OP_ALL)
-    return fail(OCTHIP_ERR_UNSUPPORTED, "only OP_ALL is supported (no domain parallelisation)");
+  OCTHIP_REQUIRE(points == OCTHIP_OP_ALL || points == OCTHIP_OP_INNER || points == OCTHIP_OP_OUTER,
+                 "points=%d is none of OP_INNER, OP_OUTER, OP_ALL", points);
+  if (points != OCTHIP_OP_ALL) {
+    // the inner or the outer runs only (nl_operator_inc.F90:257-469 with op%inner / op%outer),
+    // through the run-compressed leaf: same weights, same summation order per lane
+    const int part = points == OCTHIP_OP_INNER ? 0 : 1;
+    std::vector<double> wf(op->n);
+    for (int j = 0; j < op->n; ++j) wf[j] = op->w[j] * factor;
+    return launch_ri_runs(op->n, wf.data(), op->part_nri[part], op->d_part_ri[part], op->d_part_imin[part],
+                          op->d_part_imax[part], fi, ldf, fo, (hipStream_t)stream);
+  }
   ApplyArgs a;
   std::memset(&a, 0, sizeof(a));
   stencil_args(op, factor, a);
@@ -632,6 +672,23 @@ __global__ __launch_bounds__(BLOCK) void k_operate_ri(int n, const RiWeights w, 
 }
 }  // namespace
 
+namespace octhip {
+int launch_ri_runs(int n, const double* w, int nri, const int* ri, const int* imin, const int* imax,
+                   const double* fi, int ldf, double* fo, hipStream_t s) {
+  if (nri == 0) return OCTHIP_SUCCESS;
+  // the weights travel as a kernel argument: nothing is allocated, nothing synchronised
+  RiWeights rw;
+  for (int j = 0; j < MAX_STENCIL; ++j) rw.w[j] = j < n ? w[j] : 0.0;
+  const int tshift = ldf < 6 ? ldf : 6;
+  const int runs_per_block = BLOCK >> tshift;
+  hipLaunchKernelGGL(k_operate_ri, dim3((nri + runs_per_block - 1) / runs_per_block), dim3(BLOCK), 0, s, n, rw,
+                     nri, ri, imin, imax, fi, 1 << ldf, tshift, fo);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "operate_ri_vec: %s", hipGetErrorString(e));
+  return OCTHIP_SUCCESS;
+}
+}  // namespace octhip
+
 extern "C" int octhip_operate_ri_vec(int n, const double* w, int nri, const int* ri, const int* rimap_inv,
                                      const int* rimap_inv_max, const double* fi, int ldf, double* fo,
                                      void* stream) {
@@ -640,14 +697,5 @@ extern "C" int octhip_operate_ri_vec(int n, const double* w, int nri, const int*
   OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);
   if (nri == 0) return OCTHIP_SUCCESS;
   OCTHIP_REQUIRE(w && ri && rimap_inv && rimap_inv_max && fi && fo, "null argument");
-  // the weights travel as a kernel argument: nothing is allocated, nothing synchronised
-  RiWeights rw;
-  for (int j = 0; j < MAX_STENCIL; ++j) rw.w[j] = j < n ? w[j] : 0.0;
-  const int tshift = ldf < 6 ? ldf : 6;
-  const int runs_per_block = BLOCK >> tshift;
-  hipLaunchKernelGGL(k_operate_ri, dim3((nri + runs_per_block - 1) / runs_per_block), dim3(BLOCK), 0,
-                     (hipStream_t)stream, n, rw, nri, ri, rimap_inv, rimap_inv_max, fi, 1 << ldf, tshift, fo);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "operate_ri_vec: %s", hipGetErrorString(e));
-  return OCTHIP_SUCCESS;
+  return launch_ri_runs(n, w, nri, ri, rimap_inv, rimap_inv_max, fi, ldf, fo, (hipStream_t)stream);
 }

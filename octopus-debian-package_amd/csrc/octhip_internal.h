@@ -96,6 +96,13 @@ struct octhip_nl_operator_s {
   // run-compressed indexing (Octopus's own ri/rimap)
   int* d_ri = nullptr;                   // nri*n
   int* d_rimap = nullptr;                // np
+  // runs split as nl_operator_build does for domain parallelism (src/grid/nl_operator.F90:
+  // 457-514): [0] inner runs (every neighbour an inner point), [1] outer runs; ri rows and
+  // half-open point ranges [imin, imax) of each part
+  int part_nri[2] = {0, 0};
+  int* d_part_ri[2] = {nullptr, nullptr};
+  int* d_part_imin[2] = {nullptr, nullptr};
+  int* d_part_imax[2] = {nullptr, nullptr};
   // host copy of lattice coordinates (3*np_part), kept for schedules
   std::vector<int> coords;
   int xcd_chunk = 32;                    // gather kernel: blockIdx -> tile remap granularity

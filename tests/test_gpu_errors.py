@@ -36,8 +36,8 @@ def test_apply_rejects_bad_arguments(gpu):
     op = api.NLOperatorDevice(s.op, star_points(4))
     fi = api.Batch(4, s.np_part, False)
     fo = api.Batch(4, s.np, False)
-    with pytest.raises(_lib.OcthipError, match="OP_ALL"):
-        api.nl_operator_operate_batch(op, fi, fo, points=_lib.OP_INNER)
+    with pytest.raises(_lib.OcthipError, match="OP_INNER, OP_OUTER, OP_ALL"):
+        api.nl_operator_operate_batch(op, fi, fo, points=0)
     with pytest.raises(_lib.OcthipError, match="in place"):
         _lib.call("octhip_nl_operator_operate_batch", op.handle, fi.ptr(), fi.ptr(), fi.ldf, 1.0, _lib.OP_ALL,
                   None)

@@ -157,3 +157,22 @@ def test_periodic_handles_refuse_fused_paths(gpu, cell):
     b = api.Batch.from_states(states(s, 4, 12), s.np_part)
     with pytest.raises(api._lib.OcthipError):
         api.TaylorGraph(api.Exponential(4), hm, [b], 0.01)
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gamma_point_real_states(gpu, cell, kernel):
+    """Real states at the Gamma point: boundary rows copied from their images (no phase),
+    then the plain H-apply on the filled rows (the oracle's apply with set_bc off)."""
+    print(f"[periodic real states kernel={kernel}]", flush=True)
+    s = cell
+    hm = make(s, kernel=kernel, phases=False)
+    z = np.ascontiguousarray(states(s, 8, 13).real)
+    psib = api.Batch.from_states(z, s.np_part)
+    out = api.Batch(8, s.np, False)
+    api.hamiltonian_elec_apply_batch(hm, psib, out)
+    gpu.cuda.synchronize()
+    ref_in = workloads.pack(z, s.np_part)
+    ref_in[s.np:] = ref_in[s.bnd_src]
+    ref = O.Hamiltonian(s.op, s.v, s.proj).apply(ref_in.copy(), set_bc=False)
+    assert max_rel_err(out.ff.cpu().numpy(), ref) <= 1e-12
+    assert np.array_equal(psib.ff.cpu().numpy()[s.np:], ref_in[s.np:])
