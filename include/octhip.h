@@ -461,6 +461,12 @@ int octhip_hamiltonian_autotune_pack(octhip_hamiltonian_t hm, int nst, int reps,
 int octhip_tuning_save(const char* path);
 int octhip_tuning_load(const char* path);
 
+/* Test support, no counterpart in the reference: the allocator hook of the failure-path
+ * tests.  After `after` more successful device allocations every allocation of the
+ * library fails with OCTHIP_ERR_ALLOC ("injected allocation failure"), as a
+ * hipErrorOutOfMemory would; after < 0 turns the hook off (the default). */
+int octhip_inject_alloc_failure(long after);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,7 +62,9 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
         list(ex.map(_run, cmds))
     out = LIBDIR / LIBNAME
     tmp = LIBDIR / (LIBNAME + ".tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    # roctx: the library's ranges (profiling regions, C-ABI entry points) for rocprofv3 --marker-trace
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+          "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")

@@ -271,6 +271,7 @@ extern "C" int octhip_adjoint_destroy(octhip_adjoint_t a) {
 }
 
 extern "C" int octhip_adjoint_forward(octhip_adjoint_t a, double* psi, const double* eps, void* stream) {
+  OCTHIP_RANGE("octhip_adjoint_forward");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(a && psi && eps, "null argument");
   hipStream_t s = (hipStream_t)stream;
@@ -432,6 +433,7 @@ int backward_run(octhip_adjoint_s* a, double* chi, double* dj_deps, double* dj_d
 
 extern "C" int octhip_adjoint_backward(octhip_adjoint_t a, double* chi, double* dj_deps, double* dj_dv,
                                        void* stream) {
+  OCTHIP_RANGE("octhip_adjoint_backward");
   const int rc = backward_run(a, chi, dj_deps, dj_dv, stream);
   if (!a) return rc;
   // The Hamiltonian's work set of the recompute stream is not kept between backwards, so

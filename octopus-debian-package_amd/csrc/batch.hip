@@ -77,6 +77,7 @@ using namespace octhip;
 
 extern "C" int octhip_batch_pack(const double* unpacked, size_t ld, int nst, int np, int is_complex, double* packed,
                                  int ldf, int first, void* stream) {
+  OCTHIP_RANGE("octhip_batch_pack");
   int rc = check_pack_args(unpacked, packed, ld, nst, np, is_complex, ldf, first);
   if (rc || nst == 0 || np == 0) return rc;
   const dim3 grid((unsigned)((np + TP - 1) / TP), (unsigned)((nst + TS - 1) / TS));
@@ -93,6 +94,7 @@ extern "C" int octhip_batch_pack(const double* unpacked, size_t ld, int nst, int
 
 extern "C" int octhip_batch_unpack(const double* packed, int ldf, int first, int nst, int np, int is_complex,
                                    double* unpacked, size_t ld, void* stream) {
+  OCTHIP_RANGE("octhip_batch_unpack");
   int rc = check_pack_args(packed, unpacked, ld, nst, np, is_complex, ldf, first);
   if (rc || nst == 0 || np == 0) return rc;
   const dim3 grid((unsigned)((np + TP - 1) / TP), (unsigned)((nst + TS - 1) / TS));

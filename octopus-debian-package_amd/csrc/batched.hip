@@ -405,6 +405,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
 }
 
 extern "C" int octhip_batched_apply(octhip_batched_t b, void* stream) {
+  OCTHIP_RANGE("octhip_batched_apply");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(b != nullptr, "null handle");
   for (const Pair& p : b->pairs) OCTHIP_REQUIRE(p.hpsi != nullptr, "batched_apply: created without hpsi");
@@ -415,6 +416,7 @@ extern "C" int octhip_batched_apply(octhip_batched_t b, void* stream) {
 }
 
 extern "C" int octhip_batched_taylor_step(octhip_batched_t b, void* stream) {
+  OCTHIP_RANGE("octhip_batched_taylor_step");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(b != nullptr, "null handle");
   hipStream_t s = (hipStream_t)stream;

@@ -539,6 +539,7 @@ extern "C" int octhip_hamiltonian_trim(octhip_hamiltonian_t h) {
 
 extern "C" int octhip_hamiltonian_apply_batch(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
                                               int is_complex, int terms, int set_bc, void* stream) {
+  OCTHIP_RANGE("octhip_hamiltonian_apply_batch");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
   OCTHIP_REQUIRE(psi && hpsi && psi != hpsi, "bad batch pointers");
@@ -614,6 +615,7 @@ int set_field(octhip_hamiltonian_s* h, const double field[3], hipStream_t s) {
 
 extern "C" int octhip_exponential_taylor_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat,
                                                int order, int imag_time, void* stream) {
+  OCTHIP_RANGE("octhip_exponential_taylor_batch");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h && psi, "null argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
@@ -789,6 +791,7 @@ void drop_workset(octhip_hamiltonian_s* h, hipStream_t s) {
 }  // namespace octhip
 
 extern "C" int octhip_taylor_graph_launch(octhip_taylor_graph_t g, void* stream) {
+  OCTHIP_RANGE("octhip_taylor_graph_launch");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(g != nullptr, "null graph");
   ProfScope prof("taylor_graph", (hipStream_t)stream);
@@ -1126,6 +1129,7 @@ extern "C" int octhip_exponential_apply_batch(octhip_hamiltonian_t h, double* ps
                                               int order, int imag_time, double* psi2, double deltat2,
                                               double* inh, const double* weights, double* density,
                                               void* stream) {
+  OCTHIP_RANGE("octhip_exponential_apply_batch");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h && psi, "null argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "Taylor needs a complex batch (ldf in 1..%d)", MAX_LDF);
@@ -1143,6 +1147,7 @@ extern "C" int octhip_exponential_apply_batch(octhip_hamiltonian_t h, double* ps
 
 extern "C" int octhip_density_accumulate(const double* psi, int ldf, int np, int is_complex, const double* weights,
                                          double* density, void* stream) {
+  OCTHIP_RANGE("octhip_density_accumulate");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(psi && weights && density, "null argument");
   OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);
@@ -1200,6 +1205,7 @@ int check_driver_args(octhip_hamiltonian_s* h, double* const* psi, int nbatch, i
 extern "C" int octhip_propagator_expmid(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf, double dt,
                                         int order, const double* field_mid, const double* weights, double* density,
                                         void* stream) {
+  OCTHIP_RANGE("octhip_propagator_expmid");
   int rc = check_driver_args(h, psi, nbatch, ldf, order, weights, density);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1214,6 +1220,7 @@ extern "C" int octhip_propagator_expmid(octhip_hamiltonian_t h, double* const* p
 extern "C" int octhip_propagator_etrs(octhip_hamiltonian_t h, double* const* psi, int nbatch, int ldf, double dt,
                                      int order, const double* field_old, const double* field_new,
                                      const double* weights, double* density, void* stream) {
+  OCTHIP_RANGE("octhip_propagator_etrs");
   int rc = check_driver_args(h, psi, nbatch, ldf, order, weights, density);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -1278,6 +1285,7 @@ extern "C" int octhip_hamiltonian_set_phases(octhip_hamiltonian_t h, const doubl
 
 extern "C" int octhip_hamiltonian_apply_batch_phase(octhip_hamiltonian_t h, double* psi, double* hpsi, int ldf,
                                                     int terms, int set_bc, int psi_has_phase, void* stream) {
+  OCTHIP_RANGE("octhip_hamiltonian_apply_batch_phase");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
   OCTHIP_REQUIRE(psi && hpsi && psi != hpsi, "bad batch pointers");

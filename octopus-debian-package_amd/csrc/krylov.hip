@@ -220,6 +220,7 @@ void drop_krylov(octhip_hamiltonian_s* h, hipStream_t s, bool all) {
 extern "C" int octhip_exponential_lanczos_batch(octhip_hamiltonian_t h, double* psi, int ldf, int nst, double deltat,
                                                 int imag_time, int order, double lanczos_tol, const double* inh,
                                                 double vol, int* iterations, double* residual, void* stream) {
+  OCTHIP_RANGE("octhip_exponential_lanczos_batch");
   int rc = check_exp_args(h, psi, ldf, order);
   if (rc) return rc;
   const int L = 1 << ldf, ncol = L / 2;
@@ -337,6 +338,7 @@ extern "C" int octhip_exponential_lanczos_batch(octhip_hamiltonian_t h, double* 
 extern "C" int octhip_exponential_cheby_batch(octhip_hamiltonian_t h, double* psi, int ldf, double deltat, int order,
                                               double spectral_middle_point, double spectral_half_span,
                                               void* stream) {
+  OCTHIP_RANGE("octhip_exponential_cheby_batch");
   int rc = check_exp_args(h, psi, ldf, order);
   if (rc) return rc;
   OCTHIP_REQUIRE(spectral_half_span > 0.0, "spectral_half_span must be positive");
@@ -393,6 +395,7 @@ extern "C" int octhip_mesh_batch_nrm2(const double* a, int ldf, int np, int is_c
 extern "C" int octhip_mesh_batch_orthogonalization(int nvec, const double* const* psi, double* phi, int ldf, int nst,
                                                    int np, double vol, int normalize, double* overlap, double* norm,
                                                    void* stream) {
+  OCTHIP_RANGE("octhip_mesh_batch_orthogonalization");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(nvec >= 0 && (nvec == 0 || psi) && phi, "bad argument");
   OCTHIP_REQUIRE(ldf >= 1 && ldf <= MAX_LDF, "complex batches only (ldf in 1..%d)", MAX_LDF);

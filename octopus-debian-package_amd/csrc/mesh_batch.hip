@@ -225,6 +225,7 @@ using namespace octhip;
 
 extern "C" int octhip_mesh_batch_dotp_vector(const double* a, const double* b, int ldf, int np, int is_complex,
                                              double vol, double* dot, void* stream) {
+  OCTHIP_RANGE("octhip_mesh_batch_dotp_vector");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(a && b && dot, "null argument");
   OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);
@@ -291,6 +292,7 @@ int scratch_of_current_device(double** out) {
 
 extern "C" int octhip_mesh_batch_dotp_vector_noalloc(const double* a, const double* b, int ldf, int np, int is_complex,
                                                      double vol, double* dot, void* stream) {
+  OCTHIP_RANGE("octhip_mesh_batch_dotp_vector_noalloc");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(a && b && dot, "null argument");
   OCTHIP_REQUIRE(ldf >= (is_complex ? 1 : 0) && ldf <= MAX_LDF, "ldf=%d outside the packed range", ldf);

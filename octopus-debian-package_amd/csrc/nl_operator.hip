@@ -618,6 +618,7 @@ int launch_ri_runs(int n, const double* w, int nri, const int* ri, const int* im
 
 extern "C" int octhip_nl_operator_operate_batch(octhip_nl_operator_t op, const double* fi, double* fo,
                                                 int ldf, double factor, int points, void* stream) {
+  OCTHIP_RANGE("octhip_nl_operator_operate_batch");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(op != nullptr, "null operator handle");
   OCTHIP_REQUIRE(fi && fo, "null batch pointer");
@@ -692,6 +693,7 @@ int launch_ri_runs(int n, const double* w, int nri, const int* ri, const int* im
 extern "C" int octhip_operate_ri_vec(int n, const double* w, int nri, const int* ri, const int* rimap_inv,
                                      const int* rimap_inv_max, const double* fi, int ldf, double* fo,
                                      void* stream) {
+  OCTHIP_RANGE("octhip_operate_ri_vec");
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(n > 0 && n <= MAX_STENCIL && nri >= 0, "bad n=%d or nri=%d", n, nri);
   OCTHIP_REQUIRE(ldf >= 0 && ldf <= MAX_LDF, "ldf=%d outside 0..%d", ldf, MAX_LDF);

@@ -46,27 +46,29 @@ const double* zero_row();
       return ::octhip::fail(OCTHIP_ERR_STATE, "octhip_init has not been called");    \
   } while (0)
 
-// RAII-free device allocation helpers: every handle frees what it owns in *_destroy.
+// Device allocation and release: every allocation of the library goes through dalloc
+// (RAII-free: every handle frees what it owns in its *_destroy).
+int dalloc(void** p, size_t bytes);
+void dfree(void* p);
+
 template <typename T>
 int upload(T** dst, const T* src, size_t count, hipStream_t s) {
   *dst = nullptr;
   if (count == 0) return OCTHIP_SUCCESS;
-  hipError_t e = hipMalloc((void**)dst, count * sizeof(T));
-  if (e != hipSuccess)
-    return fail(OCTHIP_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-  e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+  const int rc = dalloc((void**)dst, count * sizeof(T));
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "upload: %s", hipGetErrorString(e));
   return OCTHIP_SUCCESS;
 }
 
-int dalloc(void** p, size_t bytes);
-void dfree(void* p);
 
 // While a graph is being captured no profiling events are recorded.
 void set_capturing(bool on);
 
-// Device timing of one kernel launch when profiling is on (events on the launch stream).
+// Device timing of one kernel launch when profiling is on (events on the launch stream);
+// always a roctx range of the region's name around the launch (host side).
 class ProfScope {
  public:
   ProfScope(const char* region, hipStream_t s);
@@ -76,6 +78,15 @@ class ProfScope {
   int slot_ = -1;
   hipStream_t s_;
 };
+
+// roctx range around one C-ABI entry point (the reference's profiling_in/out regions,
+// src/basic/profiling.F90:503-625); rocprofv3 --marker-trace records them.
+class ApiRange {
+ public:
+  explicit ApiRange(const char* name);
+  ~ApiRange();
+};
+#define OCTHIP_RANGE(name) ::octhip::ApiRange octhip_api_range_(name)
 
 }  // namespace octhip
 

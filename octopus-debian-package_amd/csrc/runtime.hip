@@ -1,8 +1,11 @@
 // Library lifecycle, error reporting and memory entry points.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <vector>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "octhip_internal.h"
 
@@ -27,9 +30,18 @@ bool initialised() { return g_init; }
 int current_device() { return g_device; }
 const double* zero_row() { return g_zero_row; }
 
+namespace {
+// allocations left before injected failures start (octhip_inject_alloc_failure); < 0: off
+std::atomic<long> g_alloc_budget{-1};
+}  // namespace
+
 int dalloc(void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return OCTHIP_SUCCESS;
+  for (long b = g_alloc_budget.load(); b >= 0; b = g_alloc_budget.load()) {
+    if (b == 0) return fail(OCTHIP_ERR_ALLOC, "hipMalloc(%zu bytes): injected allocation failure", bytes);
+    if (g_alloc_budget.compare_exchange_weak(b, b - 1)) break;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
@@ -57,6 +69,7 @@ std::vector<ProfRec> g_recs;
 void set_capturing(bool on) { g_capturing = on; }
 
 ProfScope::ProfScope(const char* region, hipStream_t s) : s_(s) {
+  roctxRangePushA(region);
   if (!g_prof || g_capturing) return;
   if (g_only[0] && std::strcmp(g_only, region) != 0) return;
   ProfRec r{region, nullptr, nullptr};
@@ -68,7 +81,11 @@ ProfScope::ProfScope(const char* region, hipStream_t s) : s_(s) {
 
 ProfScope::~ProfScope() {
   if (slot_ >= 0) (void)hipEventRecord(g_recs[slot_].stop, s_);
+  roctxRangePop();
 }
+
+ApiRange::ApiRange(const char* name) { roctxRangePushA(name); }
+ApiRange::~ApiRange() { roctxRangePop(); }
 
 static void prof_clear() {
   for (auto& r : g_recs) {
@@ -204,5 +221,10 @@ extern "C" int octhip_memcpy_dtod(void* dst, const void* src, size_t bytes, void
 extern "C" int octhip_stream_synchronize(void* stream) {
   OCTHIP_REQUIRE_INIT();
   OCTHIP_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return OCTHIP_SUCCESS;
+}
+
+extern "C" int octhip_inject_alloc_failure(long after) {
+  octhip::g_alloc_budget.store(after < 0 ? -1 : after);
   return OCTHIP_SUCCESS;
 }

@@ -80,6 +80,7 @@ SIGNATURES = {
     "octhip_hamiltonian_autotune_pack": [_P, _I, _I, _I, _P, _IP, _DP],
     "octhip_hamiltonian_memory": [_P, C.POINTER(C.c_size_t)],
     "octhip_hamiltonian_trim": [_P],
+    "octhip_inject_alloc_failure": [C.c_long],
     "octhip_tuning_save": [C.c_char_p],
     "octhip_tuning_load": [C.c_char_p],
     "octhip_profiling_select": [C.c_char_p],

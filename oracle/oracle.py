@@ -7,6 +7,7 @@ package) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -24,7 +25,8 @@ def build() -> Path:
 
 
 _lib = None
-_path = LIB
+# ORACLE_LIB: another build of the same source (the sanitizer build of the memory-safety test)
+_path = Path(os.environ["ORACLE_LIB"]) if os.environ.get("ORACLE_LIB") else LIB
 FAST = HERE / "_fast" / "liboracle_fast.so"
 
 
