@@ -193,7 +193,9 @@ int octhip_hamiltonian_set_dipole_field(octhip_hamiltonian_t hm, const double* p
  * set_phase_corr and unset_phase_corr (src/td/propagation_ops_elec.F90:238-246); the
  * exp-mid/ETRS drivers take batches without it and set/unset it around each batch.
  * Taylor graphs, batched launches and the adjoint cover finite systems only
- * (OCTHIP_ERR_UNSUPPORTED on periodic handles).  Synchronise the device first. */
+ * (OCTHIP_ERR_UNSUPPORTED on periodic handles), and both calls are refused while the
+ * handle has Taylor graphs (they keep the launches they were captured with).
+ * Synchronise the device first. */
 int octhip_hamiltonian_set_periodic(octhip_hamiltonian_t hm, const int* bnd_src);
 int octhip_hamiltonian_set_phases(octhip_hamiltonian_t hm, const double* phase,
                                   const double* phase_corr, const double* projector_phases);

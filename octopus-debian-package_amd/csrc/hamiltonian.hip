@@ -1244,6 +1244,8 @@ extern "C" int octhip_propagator_etrs(octhip_hamiltonian_t h, double* const* psi
 extern "C" int octhip_hamiltonian_set_periodic(octhip_hamiltonian_t h, const int* bnd_src) {
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  // a captured Taylor graph keeps the zero-boundary launches it was built with
+  OCTHIP_REQUIRE(h->graphs.empty(), "destroy the Taylor graphs of this Hamiltonian before changing its boundaries");
   const int np = h->np, nb = h->kin->np_part - h->np;
   (void)hipDeviceSynchronize();  // launches in flight may read the old map
   dfree(h->d_bnd_src);
@@ -1265,6 +1267,7 @@ extern "C" int octhip_hamiltonian_set_phases(octhip_hamiltonian_t h, const doubl
                                              const double* projector_phases) {
   OCTHIP_REQUIRE_INIT();
   OCTHIP_REQUIRE(h != nullptr, "null hamiltonian handle");
+  OCTHIP_REQUIRE(h->graphs.empty(), "destroy the Taylor graphs of this Hamiltonian before setting Bloch phases");
   const int np_part = h->kin->np_part, nb = np_part - h->np;
   (void)hipDeviceSynchronize();
   for (double** p : {&h->d_phase, &h->d_phase_corr, &h->d_pphase}) {

@@ -176,3 +176,23 @@ def test_gamma_point_real_states(gpu, cell, kernel):
     ref = O.Hamiltonian(s.op, s.v, s.proj).apply(ref_in.copy(), set_bc=False)
     assert max_rel_err(out.ff.cpu().numpy(), ref) <= 1e-12
     assert np.array_equal(psib.ff.cpu().numpy()[s.np:], ref_in[s.np:])
+
+
+def test_boundaries_cannot_change_under_a_graph(gpu):
+    """A Taylor graph keeps its zero-boundary launches: changing the handle's boundaries or
+    phases while it exists is refused, and allowed again once it is destroyed."""
+    import gc
+    s = PER.periodic_system(n=16, spacing=0.3, natoms=1, nwells=2, rc=1.2, seed=77)
+    from octhip.stencil import star_points
+    hm = api.HamiltonianDevice(api.NLOperatorDevice(s.op, star_points(4)), s.v, s.proj)
+    b = api.Batch.from_states(states(s, 4, 14), s.np_part)
+    g = api.TaylorGraph(api.Exponential(4), hm, [b], 0.01)
+    with pytest.raises(api._lib.OcthipError, match="Taylor graphs"):
+        hm.set_periodic(s.bnd_src)
+    phase, corr = s.phases(K)
+    with pytest.raises(api._lib.OcthipError, match="Taylor graphs"):
+        hm.set_phases(phase, corr, s.pphase_of(K))
+    del g
+    gc.collect()
+    hm.set_periodic(s.bnd_src)
+    hm.set_phases(phase, corr, s.pphase_of(K))
