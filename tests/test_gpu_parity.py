@@ -235,3 +235,24 @@ def test_operate_ri_vec_stateless_bit_exact(gpu, ldf):
     gpu.cuda.synchronize()
     ref = O.operate_ri_vec(op, f, -0.5)
     assert np.array_equal(fo.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_widest_rows(gpu, kernel):
+    """512 complex states in one batch (1024 lanes, the longest row the library takes): the
+    H-apply and one Taylor-4 step against the oracle, through the gather and the z-march
+    kernels (64 lane chunks of 16)."""
+    print(f"[widest rows kernel={kernel}]", flush=True)
+    s = small_sphere(radius=2.2, spacing=0.3, natoms=2, rc=1.0)
+    z = states(s, 512)
+    got, ref = run_h(gpu, s, z, kernel=kernel), oracle_h(s, z)
+    assert got.shape[1] == 1024
+    assert max_rel_err(got, ref) <= 1e-12
+    hm = api.HamiltonianDevice.from_system(s)
+    hm.tune(kernel=kernel)
+    psib = api.Batch.from_states(z, s.np_part)
+    api.Exponential(4).apply_batch(hm, psib, 0.01)
+    gpu.cuda.synchronize()
+    refp = workloads.pack(z, s.np_part)
+    O.Hamiltonian(s.op, s.v, s.proj).taylor(refp, 0.01, 4)
+    assert max_rel_err(psib.ff.cpu().numpy()[: s.np], refp[: s.np]) <= 1e-12
