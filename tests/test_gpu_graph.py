@@ -58,6 +58,34 @@ def test_taylor_graph_replays_bitwise(gpu, kernel, lpt):
     assert max_rel_err(g_b[0].ff.cpu().numpy()[: s.np], psi[: s.np]) <= 1e-11
 
 
+@pytest.mark.parametrize("order,imag", [(1, False), (2, True), (3, False), (6, False), (4, True)])
+def test_taylor_graph_other_orders_and_imaginary_time(gpu, order, imag):
+    """Graphs of other Taylor orders (the deferred accumulation covers orders 2-4, order 1
+    and orders above 4 accumulate order by order) and in imaginary time replay bitwise as
+    the eager steps, which match the oracle."""
+    print(f"[taylor graph order={order} imag={imag}]", flush=True)
+    s = small_sphere(radius=3.0, spacing=0.3, natoms=3, rc=1.2)
+    hm = api.HamiltonianDevice.from_system(s)
+    te = api.Exponential(order)
+    dt = -0.01 if imag else 0.01
+    g_b = _batches(s, [8, 4])
+    e_b = [b.clone() for b in g_b]
+    graph = api.TaylorGraph(te, hm, g_b, dt, imag_time=imag)
+    for rep in range(3):
+        graph.launch()
+        for b in e_b:
+            te.apply_batch(hm, b, dt, imag_time=imag)
+        gpu.cuda.synchronize()
+        for gb, eb in zip(g_b, e_b):
+            assert gpu.equal(gb.ff[: s.np], eb.ff[: s.np]), f"order {order}: replay {rep} differs"
+    oh = O.Hamiltonian(s.op, s.v, s.proj)
+    z = workloads.orthonormalise(workloads.gaussian_envelope_states(s, 8, width=2.0, first=3), s.mesh.vol_pp)
+    psi = workloads.pack(z, s.np_part)
+    for _ in range(3):
+        oh.taylor(psi, dt, order, imag)
+    assert max_rel_err(g_b[0].ff.cpu().numpy()[: s.np], psi[: s.np]) <= 1e-11
+
+
 def test_taylor_graph_sees_field_updates(gpu):
     print("[taylor graph field]", flush=True)
     s = small_sphere(radius=3.5, spacing=0.3, natoms=3, rc=1.2)
