@@ -378,8 +378,6 @@ extern "C" int octhip_hamiltonian_create(octhip_nl_operator_t kinetic, double ma
     if (!rc) rc = upload(&h->d_maps, maps, total_points, s);
     if (!rc) rc = upload(&h->d_scals, scals, nproj, s);
     if (!rc) rc = upload(&h->d_pt, pt.data(), pt.size(), s);
-    if (!rc) rc = upload(&h->d_pos, pos.data(), pos.size(), s);
-    if (!rc) rc = upload(&h->d_ent, ent.data(), ent.size(), s);
     if (!rc) rc = upload(&h->d_mat_pbase, pbase.data(), pbase.size(), s);
     if (!rc) rc = upload(&h->d_mat_npj, npj.data(), npj.size(), s);
     if (!rc) rc = upload(&h->d_chunks, chunks.data(), chunks.size(), s);
@@ -410,8 +408,6 @@ extern "C" int octhip_hamiltonian_destroy(octhip_hamiltonian_t h) {
   dfree(h->d_maps);
   dfree(h->d_scals);
   dfree(h->d_pt);
-  dfree(h->d_pos);
-  dfree(h->d_ent);
   dfree(h->d_mat_pbase);
   dfree(h->d_mat_npj);
   dfree(h->d_chunks);
@@ -500,7 +496,7 @@ extern "C" int octhip_hamiltonian_memory(octhip_hamiltonian_t h, size_t* bytes) 
   size_t data = 0, work = 0, graph = 0, old = 0;
   for (const void* p : {(const void*)h->d_v, (const void*)h->d_xyz, (const void*)h->d_veff,
                         (const void*)h->d_matrices, (const void*)h->d_maps, (const void*)h->d_scals,
-                        (const void*)h->d_pt, (const void*)h->d_pos, (const void*)h->d_ent,
+                        (const void*)h->d_pt,
                         (const void*)h->d_mat_pbase, (const void*)h->d_mat_npj, (const void*)h->d_chunks,
                         (const void*)h->d_mat_chunk0, (const void*)h->d_mat_info, (const void*)h->d_proj_mat,
                         (const void*)h->d_kslot, (const void*)h->d_multi_pos, (const void*)h->d_multi_g,

@@ -174,8 +174,6 @@ struct octhip_hamiltonian_s {
   int* d_maps = nullptr;        // 1-based
   double* d_scals = nullptr;
   double* d_pt = nullptr;       // total_points x pjs (row-major, transposed blocks)
-  int* d_pos = nullptr;         // np+1 CSR of sphere points per mesh point
-  int2* d_ent = nullptr;        // (global sphere point, matrix)
   int* d_mat_pbase = nullptr;   // nmat: offset of the matrix's projections
   int* d_mat_npj = nullptr;     // nmat: nprojs
   // bra chunking: chunk c = (matrix, first sphere point, last+1)
