@@ -351,7 +351,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
       k.total_points = h->total_points;
       ka.push_back(k);
       kb.push_back(b->nl ? h->nchunk * (b->mfma ? p.L / 16 : 1) : 0);
-      sb.push_back(blocks_for((size_t)k.nmulti * p.L, 8192));
+      sb.push_back(blocks_for((size_t)k.nmulti * ((p.L & 1) ? p.L : p.L / 2), 8192));
     }
     for (size_t i = 0; i < b->pairs.size() && b->nl && !rc; ++i)
       if (!(ra[i].partial && ra[i].proj && ka[i].proj && ka[i].kbuf))

@@ -119,13 +119,17 @@ __global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) { bra_plai
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 
-// One workgroup per (chunk, lane tile of 16 lanes): grids of nchunk * L/16 blocks keep
-// every CU busy; the 4 waves take interleaved groups of 4 sphere points and are summed
-// through LDS in a fixed order.  Every load is issued unconditionally (clamped indices,
-// zero operands past the chunk end) so the compiler counts them exactly.
-__device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int lt) {
+// One workgroup per (chunk, lane tile of 16 TW lanes): grids of nchunk * L/(16 TW) blocks
+// keep every CU busy; the 4 waves take interleaved groups of 4 sphere points and are
+// summed through LDS in a fixed order.  Every load is issued unconditionally (clamped
+// indices, zero operands past the chunk end) so the compiler counts them exactly.
+// TW = 2: lane ii of a wave loads the 16 bytes of columns 2ii and 2ii+1 of its row (one
+// complex value) and feeds two MFMAs, one per column, that share the projector operand:
+// each column's sum is the TW = 1 sum, in the same order.
+constexpr int BRA_RED = 4 * 16 * 16;  // LDS doubles per accumulator of the wave reduction
+template <int TW>
+__device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int lt, double (*red)[BRA_RED]) {
   constexpr int KB = 16;                    // MFMA steps (4 sphere points each) per load block
-  __shared__ double red[4 * 16 * 16];
   const BraArgs& b = m.b;
   const int4 ch = b.chunks[c];
   const int4 mi = b.mat_info[ch.x];
@@ -133,14 +137,17 @@ __device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int l
   const int wave = threadIdx.x >> 6;
   const int ln = threadIdx.x & 63;
   const int kk = ln >> 4, ii = ln & 15;
-  const int col = lt * 16 + ii;
+  const int col = lt * 16 * TW + ii * TW;   // first column of this lane
   for (int jb = 0; jb < npj; jb += 16) {
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    d4 acc[TW];
+#pragma unroll
+    for (int w = 0; w < TW; ++w) acc[w] = d4{0.0, 0.0, 0.0, 0.0};
     const int jc = min(jb + ii, npj - 1);
     const bool jok = jb + ii < npj;
     for (int q0 = ch.y + 4 * wave; q0 < ch.z; q0 += KB * 16) {
       int row[KB];
-      double av[KB], bv[KB];
+      double av[KB];
+      Vec<TW> bv[KB];
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const int qk = q0 + 16 * k + kk;
@@ -149,37 +156,52 @@ __device__ __forceinline__ void bra_mfma_body(const BraMfmaArgs& m, int c, int l
         av[k] = m.pt[(size_t)(mapoff + qs) * m.pjs + jc];
       }
 #pragma unroll
-      for (int k = 0; k < KB; ++k) bv[k] = b.psi[(size_t)row[k] * b.L + col];
+      for (int k = 0; k < KB; ++k) bv[k] = ldv<TW>(b.psi + (size_t)row[k] * b.L + col);
       if (b.pphase) {
-        // Bloch phase of each sphere point; the other lane of the (re, im) pair is the
-        // neighbouring thread's (lanes ii and ii ^ 1 hold the same point)
+        // Bloch phase of each sphere point on the (re, im) pair: with TW = 2 the lane
+        // holds both, otherwise the other one is the neighbouring thread's (lanes ii and
+        // ii ^ 1 hold the same point)
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
           const int qk = q0 + 16 * k + kk;
           const size_t pi2 = 2 * (size_t)(mapoff + (qk < ch.z ? qk : ch.y));
-          const double xq = __shfl_xor(bv[k], 1);
-          bv[k] = zmul_lane(b.pphase[pi2], b.pphase[pi2 + 1], bv[k], xq, (col & 1) != 0);
+          if constexpr (TW == 2) {
+            const double re = zmul_lane(b.pphase[pi2], b.pphase[pi2 + 1], bv[k].v[0], bv[k].v[1], false);
+            const double im = zmul_lane(b.pphase[pi2], b.pphase[pi2 + 1], bv[k].v[1], bv[k].v[0], true);
+            bv[k].v[0] = re;
+            bv[k].v[1] = im;
+          } else {
+            const double xq = __shfl_xor(bv[k].v[0], 1);
+            bv[k].v[0] = zmul_lane(b.pphase[pi2], b.pphase[pi2 + 1], bv[k].v[0], xq, (col & 1) != 0);
+          }
         }
       }
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const bool ok = q0 + 16 * k + kk < ch.z;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64((ok && jok) ? av[k] : 0.0, ok ? bv[k] : 0.0, acc, 0, 0, 0);
+#pragma unroll
+        for (int w = 0; w < TW; ++w)
+          acc[w] = __builtin_amdgcn_mfma_f64_16x16x4f64((ok && jok) ? av[k] : 0.0, ok ? bv[k].v[w] : 0.0, acc[w],
+                                                        0, 0, 0);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[(wave * 16 + 4 * r + kk) * 16 + ii] = acc[r];
+    for (int w = 0; w < TW; ++w)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][(wave * 16 + 4 * r + kk) * 16 + ii] = acc[w][r];
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int jl = 4 * r + kk;
-        const double t = (red[(0 * 16 + jl) * 16 + ii] + red[(1 * 16 + jl) * 16 + ii]) +
-                         (red[(2 * 16 + jl) * 16 + ii] + red[(3 * 16 + jl) * 16 + ii]);
-        const int j = jb + jl;
-        if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col] = t;
-      }
+      for (int w = 0; w < TW; ++w)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int jl = 4 * r + kk;
+          const double t = (red[w][(0 * 16 + jl) * 16 + ii] + red[w][(1 * 16 + jl) * 16 + ii]) +
+                           (red[w][(2 * 16 + jl) * 16 + ii] + red[w][(3 * 16 + jl) * 16 + ii]);
+          const int j = jb + jl;
+          if (j < npj) b.partial[((size_t)c * b.pjmax + j) * b.L + col + w] = t;
+        }
     }
   }
 }
@@ -194,8 +216,15 @@ __global__ __launch_bounds__(BLOCK) void k_bra_mfma(const BraMfmaArgs m) {
       m.zf[m.zfirst + t] = 0.0;
     return;
   }
-  const int b = (int)blockIdx.x - nz, nlt = m.b.L >> 4;
-  bra_mfma_body(m, b / nlt, b % nlt);
+  __shared__ double red[2][BRA_RED];
+  const int b = (int)blockIdx.x - nz;
+  if ((m.b.L & 31) == 0) {
+    const int nlt = m.b.L >> 5;
+    bra_mfma_body<2>(m, b / nlt, b % nlt, red);
+  } else {
+    const int nlt = m.b.L >> 4;
+    bra_mfma_body<1>(m, b / nlt, b % nlt, red);
+  }
 }
 
 // proj[i, l] = scal[i] * sum_{chunks of matrix(i)} partial[chunk, i - pbase, l]
@@ -243,7 +272,8 @@ int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s) {
 int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hipStream_t s, double* zf,
                     size_t zfirst, size_t zcount) {
   if (nchunk <= 0) return OCTHIP_SUCCESS;
-  BraMfmaArgs m{b, pt, pjs, zf, zfirst, zf ? zcount : 0, nchunk * (b.L >> 4)};
+  // 32-lane tiles (two columns per lane) when the row length allows, else 16-lane tiles
+  BraMfmaArgs m{b, pt, pjs, zf, zfirst, zf ? zcount : 0, nchunk * ((b.L & 31) == 0 ? b.L >> 5 : b.L >> 4)};
   const int zblocks = zf ? (int)std::min<size_t>((zcount + BLOCK - 1) / BLOCK, 2048) : 0;
   ProfScope prof("k_bra", s);
   hipLaunchKernelGGL(k_bra_mfma, dim3(m.nbra + zblocks), dim3(BLOCK), 0, s, m);
@@ -343,24 +373,37 @@ __device__ __forceinline__ void ket_plain_body(const KetArgs& k, int c) {
   }
 }
 
-__device__ __forceinline__ void ket_sum_body(const KetArgs& k, int bid, int nblk) {
-  const size_t total = (size_t)k.nmulti * k.L;
+// one thread per (multi-sphere mesh point, LPT consecutive lanes of its row)
+template <int LPT>
+__device__ __forceinline__ void ket_sum_lanes(const KetArgs& k, int bid, int nblk) {
+  const int lp = k.L / LPT;
+  const size_t total = (size_t)k.nmulti * lp;
   for (size_t t = bid * (size_t)BLOCK + threadIdx.x; t < total; t += (size_t)nblk * BLOCK) {
-    const int m = (int)(t / k.L);
-    const int l = (int)(t - (size_t)m * k.L);
+    const int m = (int)(t / lp);
+    const int l = (int)(t - (size_t)m * lp) * LPT;
     const int e0 = k.multi_pos[m], e1 = k.multi_pos[m + 1];
-    double s = 0.0;
+    Vec<LPT> s;
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) s.v[q] = 0.0;
     // rows in groups of 4 loaded together (clamped indices), added in entry order
     for (int e = e0; e < e1; e += 4) {
-      double v[4];
+      Vec<LPT> v[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = k.kbuf[(size_t)k.multi_g[min(e + u, e1 - 1)] * k.L + l];
+      for (int u = 0; u < 4; ++u) v[u] = ldv<LPT>(k.kbuf + (size_t)k.multi_g[min(e + u, e1 - 1)] * k.L + l);
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        if (e + u < e1) s = s + v[u];
+        if (e + u < e1) {
+#pragma unroll
+          for (int q = 0; q < LPT; ++q) s.v[q] = s.v[q] + v[u].v[q];
+        }
     }
-    k.kbuf[((size_t)k.total_points + m) * k.L + l] = s;
+    stv<LPT>(k.kbuf + ((size_t)k.total_points + m) * k.L + l, s);
   }
+}
+
+__device__ __forceinline__ void ket_sum_body(const KetArgs& k, int bid, int nblk) {
+  if (k.L & 1) ket_sum_lanes<1>(k, bid, nblk);
+  else ket_sum_lanes<2>(k, bid, nblk);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ket_plain(const KetArgs k) { ket_plain_body(k, blockIdx.x); }
@@ -380,7 +423,8 @@ int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(OCTHIP_ERR_HIP, "k_ket launch: %s", hipGetErrorString(e));
   }
-  const size_t total = (size_t)k.nmulti * k.L;
+  // one thread per lane pair of each multi-sphere point (per lane for odd row lengths)
+  const size_t total = (size_t)k.nmulti * ((k.L & 1) ? k.L : k.L / 2);
   if (total == 0) return OCTHIP_SUCCESS;
   const int grid = (int)std::min<size_t>((total + BLOCK - 1) / BLOCK, 8192);
   ProfScope prof("k_ket_sum", s);
@@ -424,8 +468,9 @@ __device__ __forceinline__ int segment_of(const int* __restrict__ off, int nseg,
 __global__ __launch_bounds__(BLOCK) void k_bra_mfma_multi(const BraMfmaArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
+  __shared__ double red[1][BRA_RED];
   const int b = blockIdx.x - off[i], nlt = a[i].b.L >> 4;
-  bra_mfma_body(a[i], b / nlt, b % nlt);
+  bra_mfma_body<1>(a[i], b / nlt, b % nlt, red);
 }
 __global__ __launch_bounds__(BLOCK) void k_bra_plain_multi(const BraMfmaArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
