@@ -288,21 +288,25 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
 // accumulator element r of lane (kk, ii) is D[4r+kk][ii].
 // One workgroup per (chunk, lane tile of 16); the 4 waves take the chunk's groups of 16
 // sphere points in turn.  Loads are unconditional (clamped indices, zero operands).
-__device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
+// TW = 2: 32-lane tiles, lane ii computes columns 2ii and 2ii+1 with two MFMAs that share
+// the projector operand and stores them as one 16-byte piece (one complex value); each
+// column's sum is the TW = 1 sum, in the same order.
+template <int TW>
+__device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt, double* sproj) {
   const int4 ch = k.chunks[c];
   const int4 mi = k.mat_info[ch.x];
   const int npj = mi.y, mapoff = mi.w;
   const int pb = k.mat_pbase[ch.x];
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int kk = ln >> 4, ii = ln & 15;
-  const int col = lt * 16 + ii;
+  constexpr int TL = 16 * TW;               // lanes of the tile
+  const int col = lt * TL + ii * TW;        // first column of this lane
   const int ngrp = (ch.z - ch.y + 15) >> 4;
-  __shared__ double sproj[KET_FUSE_PJMAX * 16];
   if (k.partial) {
-    // proj[pb + j, lt * 16 + i] for j < npj, i < 16: the bra_reduce_body sum, same order
+    // proj[pb + j, lt * TL + i] for j < npj, i < TL: the bra_reduce_body sum, same order
     const int c0 = k.mat_chunk0[ch.x], c1 = k.mat_chunk0[ch.x + 1];
-    for (int t = threadIdx.x; t < npj * 16; t += BLOCK) {
-      const int j = t >> 4, l = lt * 16 + (t & 15);
+    for (int t = threadIdx.x; t < npj * TL; t += BLOCK) {
+      const int j = t / TL, l = lt * TL + (t % TL);
       double s = 0.0;
       for (int c = c0; c < c1; c += 32) {
         double v[32];
@@ -320,35 +324,59 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt) {
     const int pt0 = ch.y + g * 16;
     const int q = pt0 + ii;
     const int qs = q < ch.z ? q : ch.y;
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    d4 acc[TW];
+#pragma unroll
+    for (int w = 0; w < TW; ++w) acc[w] = d4{0.0, 0.0, 0.0, 0.0};
     for (int jb = 0; jb < npj; jb += 4) {
       const bool jok = jb + kk < npj;
       const int jj = jok ? jb + kk : npj - 1;
       const double a = k.pt[(size_t)(mapoff + qs) * k.pjs + jj];
-      const double bb = k.partial ? sproj[jj * 16 + ii] : k.proj[(size_t)(pb + jj) * k.L + col];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((q < ch.z && jok) ? a : 0.0, jok ? bb : 0.0, acc, 0, 0, 0);
+      const Vec<TW> bb = k.partial ? ldv<TW>(sproj + jj * TL + ii * TW) : ldv<TW>(k.proj + (size_t)(pb + jj) * k.L + col);
+#pragma unroll
+      for (int w = 0; w < TW; ++w)
+        acc[w] = __builtin_amdgcn_mfma_f64_16x16x4f64((q < ch.z && jok) ? a : 0.0, jok ? bb.v[w] : 0.0, acc[w], 0, 0, 0);
     }
     if (k.pphase) {
-      // ket row of point qo times conj(its Bloch phase); lanes ii ^ 1 hold the pair
+      // ket row of point qo times conj(its Bloch phase): with TW = 2 the lane holds the
+      // (re, im) pair, otherwise lanes ii ^ 1 hold the pair
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int qo = pt0 + 4 * r + kk;
         const size_t pi2 = 2 * (size_t)(mapoff + (qo < ch.z ? qo : ch.y));
-        const double xq = __shfl_xor(acc[r], 1);
-        acc[r] = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc[r], xq, (col & 1) != 0);
+        if constexpr (TW == 2) {
+          const double re = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc[0][r], acc[1][r], false);
+          const double im = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc[1][r], acc[0][r], true);
+          acc[0][r] = re;
+          acc[1][r] = im;
+        } else {
+          const double xq = __shfl_xor(acc[0][r], 1);
+          acc[0][r] = zmul_lane(k.pphase[pi2], -k.pphase[pi2 + 1], acc[0][r], xq, (col & 1) != 0);
+        }
       }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int qo = pt0 + 4 * r + kk;
-      if (qo < ch.z) k.kbuf[(size_t)(mapoff + qo) * k.L + col] = acc[r];
+      if (qo < ch.z) {
+        Vec<TW> o;
+#pragma unroll
+        for (int w = 0; w < TW; ++w) o.v[w] = acc[w][r];
+        stv<TW>(k.kbuf + (size_t)(mapoff + qo) * k.L + col, o);
+      }
     }
   }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ket_mfma(const KetArgs k) {
-  const int nlt = k.L >> 4;
-  ket_mfma_body(k, blockIdx.x / nlt, blockIdx.x % nlt);
+  __shared__ double2 sproj2[KET_FUSE_PJMAX * 16];  // 16-byte aligned: read as lane pairs
+  double* const sproj = reinterpret_cast<double*>(sproj2);
+  if ((k.L & 31) == 0) {
+    const int nlt = k.L >> 5;
+    ket_mfma_body<2>(k, blockIdx.x / nlt, blockIdx.x % nlt, sproj);
+  } else {
+    const int nlt = k.L >> 4;
+    ket_mfma_body<1>(k, blockIdx.x / nlt, blockIdx.x % nlt, sproj);
+  }
 }
 
 // plain form, any row length: one thread per (sphere point, lane)
@@ -416,8 +444,8 @@ int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
                 KET_FUSE_PJMAX);
   {
     ProfScope prof("k_ket", s);
-    if (mfma && k.L >= 16)
-      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk * (k.L >> 4)), dim3(BLOCK), 0, s, k);
+    if (mfma && k.L >= 16)  // 32-lane tiles when the row length allows, else 16-lane tiles
+      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk * ((k.L & 31) == 0 ? k.L >> 5 : k.L >> 4)), dim3(BLOCK), 0, s, k);
     else
       hipLaunchKernelGGL(k_ket_plain, dim3(nchunk), dim3(BLOCK), 0, s, k);
     hipError_t e = hipGetLastError();
@@ -485,8 +513,9 @@ __global__ __launch_bounds__(BLOCK) void k_bra_reduce_multi(const BraReduceArgs*
 __global__ __launch_bounds__(BLOCK) void k_ket_mfma_multi(const KetArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
+  __shared__ double sproj[KET_FUSE_PJMAX * 16];
   const int b = blockIdx.x - off[i], nlt = a[i].L >> 4;
-  ket_mfma_body(a[i], b / nlt, b % nlt);
+  ket_mfma_body<1>(a[i], b / nlt, b % nlt, sproj);
 }
 __global__ __launch_bounds__(BLOCK) void k_ket_plain_multi(const KetArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
