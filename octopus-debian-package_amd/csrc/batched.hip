@@ -142,8 +142,9 @@ int build_bra(octhip_batched_s* b, const std::vector<const double*>& in, Seg& ou
     if (b->nl && !(m.b.psi && m.b.partial && m.b.chunks && m.b.maps && m.pt))
       return fail(OCTHIP_ERR_STATE, "batched: incomplete bra arguments for pair %zu", i);
     args.push_back(m);
-    // MFMA bra: one block per (chunk, lane tile of 16); plain bra: one per chunk
-    blocks.push_back(h->nmat > 0 ? h->nchunk * (b->mfma ? p.L / 16 : 1) : 0);
+    // MFMA bra: one block per (chunk, lane tile of 32, or of 16 when 32 does not divide
+    // the row); plain bra: one per chunk
+    blocks.push_back(h->nmat > 0 ? h->nchunk * (b->mfma ? mfma_lane_tiles(p.L) : 1) : 0);
   }
   return make_seg(args, blocks, out);
 }
@@ -350,7 +351,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
       k.nmulti = b->nl ? h->nmulti : 0;
       k.total_points = h->total_points;
       ka.push_back(k);
-      kb.push_back(b->nl ? h->nchunk * (b->mfma ? p.L / 16 : 1) : 0);
+      kb.push_back(b->nl ? h->nchunk * (b->mfma ? mfma_lane_tiles(p.L) : 1) : 0);
       sb.push_back(blocks_for((size_t)k.nmulti * ((p.L & 1) ? p.L : p.L / 2), 8192));
     }
     for (size_t i = 0; i < b->pairs.size() && b->nl && !rc; ++i)

@@ -396,6 +396,9 @@ int launch_multi(int kind, const void* args, const int* off, int nseg, int nbloc
 
 int launch_bra(const BraArgs& b, int nchunk, bool mfma, hipStream_t s);
 int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s);
+// lane tiles of the MFMA bra and ket over rows of L lanes (L >= 16): 32-lane tiles (two
+// columns per lane) when 32 divides L, else 16-lane tiles
+__host__ __device__ inline int mfma_lane_tiles(int L) { return (L & 31) == 0 ? L >> 5 : L >> 4; }
 int launch_bra_reduce(const BraReduceArgs& r, hipStream_t s);
 
 // sphere points per bra chunk

@@ -273,7 +273,7 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
                     size_t zfirst, size_t zcount) {
   if (nchunk <= 0) return OCTHIP_SUCCESS;
   // 32-lane tiles (two columns per lane) when the row length allows, else 16-lane tiles
-  BraMfmaArgs m{b, pt, pjs, zf, zfirst, zf ? zcount : 0, nchunk * ((b.L & 31) == 0 ? b.L >> 5 : b.L >> 4)};
+  BraMfmaArgs m{b, pt, pjs, zf, zfirst, zf ? zcount : 0, nchunk * mfma_lane_tiles(b.L)};
   const int zblocks = zf ? (int)std::min<size_t>((zcount + BLOCK - 1) / BLOCK, 2048) : 0;
   ProfScope prof("k_bra", s);
   hipLaunchKernelGGL(k_bra_mfma, dim3(m.nbra + zblocks), dim3(BLOCK), 0, s, m);
@@ -445,7 +445,7 @@ int launch_ket(const KetArgs& k, int nchunk, bool mfma, hipStream_t s) {
   {
     ProfScope prof("k_ket", s);
     if (mfma && k.L >= 16)  // 32-lane tiles when the row length allows, else 16-lane tiles
-      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk * ((k.L & 31) == 0 ? k.L >> 5 : k.L >> 4)), dim3(BLOCK), 0, s, k);
+      hipLaunchKernelGGL(k_ket_mfma, dim3(nchunk * mfma_lane_tiles(k.L)), dim3(BLOCK), 0, s, k);
     else
       hipLaunchKernelGGL(k_ket_plain, dim3(nchunk), dim3(BLOCK), 0, s, k);
     hipError_t e = hipGetLastError();
@@ -496,9 +496,10 @@ __device__ __forceinline__ int segment_of(const int* __restrict__ off, int nseg,
 __global__ __launch_bounds__(BLOCK) void k_bra_mfma_multi(const BraMfmaArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
-  __shared__ double red[1][BRA_RED];
-  const int b = blockIdx.x - off[i], nlt = a[i].b.L >> 4;
-  bra_mfma_body<1>(a[i], b / nlt, b % nlt, red);
+  __shared__ double red[2][BRA_RED];
+  const int b = blockIdx.x - off[i], nlt = mfma_lane_tiles(a[i].b.L);
+  if ((a[i].b.L & 31) == 0) bra_mfma_body<2>(a[i], b / nlt, b % nlt, red);
+  else bra_mfma_body<1>(a[i], b / nlt, b % nlt, red);
 }
 __global__ __launch_bounds__(BLOCK) void k_bra_plain_multi(const BraMfmaArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
@@ -513,9 +514,11 @@ __global__ __launch_bounds__(BLOCK) void k_bra_reduce_multi(const BraReduceArgs*
 __global__ __launch_bounds__(BLOCK) void k_ket_mfma_multi(const KetArgs* __restrict__ a,
                                                           const int* __restrict__ off, int nseg) {
   const int i = segment_of(off, nseg, blockIdx.x);
-  __shared__ double sproj[KET_FUSE_PJMAX * 16];
-  const int b = blockIdx.x - off[i], nlt = a[i].L >> 4;
-  ket_mfma_body<1>(a[i], b / nlt, b % nlt, sproj);
+  __shared__ double2 sproj2[KET_FUSE_PJMAX * 16];  // 16-byte aligned: read as lane pairs
+  double* const sproj = reinterpret_cast<double*>(sproj2);
+  const int b = blockIdx.x - off[i], nlt = mfma_lane_tiles(a[i].L);
+  if ((a[i].L & 31) == 0) ket_mfma_body<2>(a[i], b / nlt, b % nlt, sproj);
+  else ket_mfma_body<1>(a[i], b / nlt, b % nlt, sproj);
 }
 __global__ __launch_bounds__(BLOCK) void k_ket_plain_multi(const KetArgs* __restrict__ a,
                                                            const int* __restrict__ off, int nseg) {
