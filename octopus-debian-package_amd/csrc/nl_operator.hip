@@ -158,7 +158,7 @@ namespace octhip {
 int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 
 // Segment length of a launch with nchunk lane chunks of lc lanes: the operator's setting,
-// or the candidate (48, 32, 24, 20, 16, 12, 8 planes) with the smallest estimated time,
+// or the candidate (48, 32, 28, 24, 20, 16, 12, 8 planes) with the smallest estimated time,
 // in plane steps of one workgroup:
 //   work  = nchunk * sum over items (planes + 8 overlap planes + ZM_PROLOGUE)
 //   time  = max(work / (CUs * g), longest item * r / g)
@@ -182,7 +182,7 @@ int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident
   const bool two_ok = resident >= 2;
   double best_t = 1e300;
   int best = 8;
-  for (int zs : {48, 32, 24, 20, 16, 12, 8}) {
+  for (int zs : {48, 32, 28, 24, 20, 16, 12, 8}) {
     const size_t shm = zmarch_shmem(lc, zs);
     if (shm > 160 * 1024) continue;
     const int r = (two_ok && shm <= 80 * 1024) ? 2 : 1;
@@ -267,30 +267,27 @@ int zmarch_items(octhip_nl_operator_s* op, int zs) {
         const int ze = zb + base + (sg < rem ? 1 : 0);
         items.push_back(make_int4(tx, ty, zb, ze));
       }
-  // Index table of every item, read by the kernel's prologue as one coalesced block: for
-  // planes P = zb-4 .. ze+3, ZS sites each (the ZNC tile columns, then the ZNH halo sites
-  // of the cross-shaped window in the kernel's slot order), the point index at the site
-  // or -1; halo entries only for the item's own planes zb .. ze-1.
+  // Index table of every item, read by the kernel's prologue as one coalesced block: the
+  // point index (or -1) at the ZNC tile columns of planes zb-4 .. ze+3, then at the ZNH
+  // halo sites of the cross-shaped window (the kernel's slot order) of planes zb .. ze-1.
   std::vector<int> tab, toff(items.size());
+  auto site = [&](long x, long y, long z) {
+    return op->mcube[(size_t)x + (size_t)op->mx * ((size_t)y + (size_t)op->my * (size_t)z)];
+  };
   for (size_t i = 0; i < items.size(); ++i) {
     const int4 it = items[i];
     toff[i] = (int)tab.size();
     const long x0 = (long)it.x * ZTX + ZR, y0 = (long)it.y * ZTY + ZR;
     for (int P = it.z - 4; P < it.w + 4; ++P)
-      for (int t = 0; t < ZS; ++t) {
+      for (int t = 0; t < ZNC; ++t) tab.push_back(site(x0 + t % ZTX, y0 + t / ZTX, P));
+    for (int P = it.z; P < it.w; ++P)
+      for (int h = 0; h < ZNH; ++h) {
         long hx, hy;
-        if (t < ZNC) {
-          hx = x0 + t % ZTX;
-          hy = y0 + t / ZTX;
-        } else {
-          const int h = t - ZNC;
-          if (h < 4 * ZTY) { hx = x0 - 4 + h % 4; hy = y0 + h / 4; }
-          else if (h < 8 * ZTY) { hx = x0 + ZTX + (h - 4 * ZTY) % 4; hy = y0 + (h - 4 * ZTY) / 4; }
-          else if (h < 8 * ZTY + 4 * ZTX) { hx = x0 + (h - 8 * ZTY) % ZTX; hy = y0 - 4 + (h - 8 * ZTY) / ZTX; }
-          else { hx = x0 + (h - 8 * ZTY - 4 * ZTX) % ZTX; hy = y0 + ZTY + (h - 8 * ZTY - 4 * ZTX) / ZTX; }
-        }
-        const bool need = t < ZNC || (P >= it.z && P < it.w);
-        tab.push_back(need ? op->mcube[(size_t)hx + (size_t)op->mx * ((size_t)hy + (size_t)op->my * P)] : -1);
+        if (h < 4 * ZTY) { hx = x0 - 4 + h % 4; hy = y0 + h / 4; }
+        else if (h < 8 * ZTY) { hx = x0 + ZTX + (h - 4 * ZTY) % 4; hy = y0 + (h - 4 * ZTY) / 4; }
+        else if (h < 8 * ZTY + 4 * ZTX) { hx = x0 + (h - 8 * ZTY) % ZTX; hy = y0 - 4 + (h - 8 * ZTY) / ZTX; }
+        else { hx = x0 + (h - 8 * ZTY - 4 * ZTX) % ZTX; hy = y0 + ZTY + (h - 8 * ZTY - 4 * ZTX) / ZTX; }
+        tab.push_back(site(hx, hy, P));
       }
   }
   if (tab.size() > (size_t)INT_MAX) return fail(OCTHIP_ERR_UNSUPPORTED, "z-march index tables too large");

@@ -40,7 +40,11 @@ constexpr int ZNH = 8 * (ZTX + ZTY);  // halo sites of the cross-shaped plane wi
 constexpr int ZS = ZNC + ZNH;         // sites of the plane window
 constexpr int ZHU = (ZNH + ZNC - 1) / ZNC;  // halo sites loaded per column thread group
 static_assert((ZTX & (ZTX - 1)) == 0 && (ZTY & (ZTY - 1)) == 0 && ZS % 4 == 0, "tile shape");
-constexpr int ZPADZ = 8;             // march-cube padding below/above the inner points in z
+constexpr int ZPADZ = 8;
+// ints of the index table of an item of zs planes: own columns of zs + 8 planes, halo sites
+// of zs planes (a multiple of 4: the table is copied as int4)
+__host__ __device__ constexpr int zmarch_tab_ints(int zs) { return (zs + 8) * ZNC + zs * ZNH; }
+static_assert(ZNC % 4 == 0 && ZNH % 4 == 0, "index table copied as int4");             // march-cube padding below/above the inner points in z
 // Prefetch depths (planes loaded ahead) of the own column values, the halo values and the
 // output-point data, and the own-value ring length (= z loop unroll, >= own depth + 5).
 // A tile's own values of plane P are the halo of its neighbours, which load them HD
@@ -112,7 +116,7 @@ constexpr int ring_above(int rq, int d) {
 struct ZArgs {
   ApplyArgs a;
   const int4* __restrict__ items;    // {tile x, tile y, first plane, end plane} (cube z)
-  const int* __restrict__ tab;       // per item: index table of planes zb-4 .. ze+3, ZS sites each
+  const int* __restrict__ tab;       // per item: index table (zmarch_tab_ints ints, layout below)
   const int* __restrict__ tab_off;   // per item: first int of its table (a multiple of 4)
   const int2* __restrict__ sched;    // blockIdx -> {item, lane chunk}
 };
@@ -204,16 +208,17 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   };
 
   const int zb = it.z, ze = it.w;
-  int* const tks = tidx + (ze - zb + 8) * ZS;  // ket slots of the own sites
-  // ---- index table of the item in LDS: ZS sites per plane (the ZNC own columns, then the
-  //      ZNH halo sites), planes zb-4 .. ze+3; halo entries exist for planes zb .. ze-1.
+  // ---- index table of the item in LDS: the ZNC own columns of planes zb-4 .. ze+3, then
+  //      the ZNH halo sites of planes zb .. ze-1 (the only planes whose halo is read).
   //      Precomputed per item, so the prologue is one coalesced block copy; index reads
   //      then cost LDS, not the in-order global-load counter.
+  const int nown = (ze - zb + 8) * ZNC;
+  int* const tks = tidx + nown + (ze - zb) * ZNH;  // ket slots of the own sites
   {
     // (through a buffer resource: the descriptors of a batched launch live in global
     // memory, and a generic pointer loaded from there would be read with flat loads that
     // the compiler must order against the LDS stores)
-    const int n4 = (ze - zb + 8) * ZS / 4;
+    const int n4 = zmarch_tab_ints(ze - zb) / 4;
     const auto r_tab = zrsrc(za.tab + za.tab_off[item], (uint32_t)n4 * 16u);
     int4* const t4 = reinterpret_cast<int4*>(tidx);
     for (int base = 0; base < n4; base += NT * 8) {
@@ -231,8 +236,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
     }
   }
   __syncthreads();
-  auto own_idx = [&](int P) { return tidx[(P - zb + 4) * ZS + col]; };
-  auto halo_idx = [&](int P, int i) { return halo_ok(i) ? tidx[(P - zb + 4) * ZS + ZNC + col + ZNC * i] : -1; };
+  auto own_idx = [&](int P) { return tidx[(P - zb + 4) * ZNC + col]; };
+  // (zb <= P < ze)
+  auto halo_idx = [&](int P, int i) { return halo_ok(i) ? tidx[nown + (P - zb) * ZNH + col + ZNC * i] : -1; };
 
   // Rings with static slots (the z loop is unrolled by RQ, so every slot index is a
   // compile-time constant and values stay in the registers their loads wrote):
@@ -278,17 +284,18 @@ __device__ __forceinline__ void zmarch_body(const ZArgs& za, const int4 it, cons
   for (int k = 0; k < HD; ++k)
 #pragma unroll
     for (int i = 0; i < ZHU; ++i) hv[k][i] = val(zb + k < ze ? halo_idx(zb + k, i) : -1);
-  // ket slots of the own sites, looked up while the first planes' values are in flight
-  // (one unconditional range-checked load per entry; -1 where the site has no point)
+  // ket slots of the own sites of the output planes zb .. ze-1 (entries (P - zb + 4) ZNC +
+  // col, as the own indices), looked up while the first planes' values are in flight (one
+  // unconditional range-checked load per entry; -1 where the site has no point)
   if constexpr (NL) {
     const auto r_ks = zrsrc(a_kslot, (uint32_t)a_np * 4u);
-    const int totk = (ze - zb + 8) * ZNC;
-    for (int base = 0; base < totk; base += NT * 8) {
+    const int totk = (ze - zb + 4) * ZNC;
+    for (int base = 4 * ZNC; base < totk; base += NT * 8) {
       int kq[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = min(base + (int)threadIdx.x + u * NT, totk - 1);
-        const int q = tidx[(e / ZNC) * ZS + (e % ZNC)];
+        const int q = tidx[e];
         const int k = (int)__builtin_amdgcn_raw_buffer_load_b32(r_ks, (uint32_t)q << 2, 0, 0);
         kq[u] = (unsigned)q < (unsigned)a_np ? k : -1;
       }
@@ -441,9 +448,10 @@ __global__ __launch_bounds__(ZNC * LC / LPT) void k_zmarch_multi(const ZArgs* __
   zmarch_body<LPT, LC, EPI, NL, true>(za, za.items[sc.y], sc.y, sc.z);
 }
 
-// dynamic LDS bytes of k_zmarch for a lane chunk and segment length
+// dynamic LDS bytes of k_zmarch for a lane chunk and segment length: two plane buffers,
+// the item's index table and the ket slots of its own sites
 inline size_t zmarch_shmem(int lc, int zseg) {
-  return sizeof(double) * 2 * ZS * lc + sizeof(int) * (size_t)(zseg + 8) * (ZS + ZNC);
+  return sizeof(double) * 2 * ZS * lc + sizeof(int) * ((size_t)zmarch_tab_ints(zseg) + (size_t)(zseg + 8) * ZNC);
 }
 
 }  // namespace octhip
