@@ -301,7 +301,16 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt, d
   const int kk = ln >> 4, ii = ln & 15;
   constexpr int TL = 16 * TW;               // lanes of the tile
   const int col = lt * TL + ii * TW;        // first column of this lane
-  const int ngrp = (ch.z - ch.y + 15) >> 4;
+  const int ngrp = (ch.z - ch.y + 15) >> 4;  // <= 16: each wave takes at most 4 groups
+  static_assert(BRA_CHUNK <= 256, "4 groups of 16 sphere points per wave");
+  // projector values of the first 4 projectors of this wave's groups, loaded before the
+  // projections are formed so that both sets of loads are in flight together
+  double apre[4];
+#pragma unroll
+  for (int gi = 0; gi < 4; ++gi) {
+    const int q = ch.y + (wave + 4 * gi) * 16 + ii;
+    apre[gi] = k.pt[(size_t)(mapoff + (q < ch.z ? q : ch.y)) * k.pjs + (kk < npj ? kk : npj - 1)];
+  }
   if (k.partial) {
     // proj[pb + j, lt * TL + i] for j < npj, i < TL: the bra_reduce_body sum, same order
     const int c0 = k.mat_chunk0[ch.x], c1 = k.mat_chunk0[ch.x + 1];
@@ -320,7 +329,10 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt, d
     }
     __syncthreads();
   }
-  for (int g = wave; g < ngrp; g += 4) {
+#pragma unroll
+  for (int gi = 0; gi < 4; ++gi) {
+    const int g = wave + 4 * gi;
+    if (g >= ngrp) break;
     const int pt0 = ch.y + g * 16;
     const int q = pt0 + ii;
     const int qs = q < ch.z ? q : ch.y;
@@ -330,7 +342,7 @@ __device__ __forceinline__ void ket_mfma_body(const KetArgs& k, int c, int lt, d
     for (int jb = 0; jb < npj; jb += 4) {
       const bool jok = jb + kk < npj;
       const int jj = jok ? jb + kk : npj - 1;
-      const double a = k.pt[(size_t)(mapoff + qs) * k.pjs + jj];
+      const double a = jb == 0 ? apre[gi] : k.pt[(size_t)(mapoff + qs) * k.pjs + jj];
       const Vec<TW> bb = k.partial ? ldv<TW>(sproj + jj * TL + ii * TW) : ldv<TW>(k.proj + (size_t)(pb + jj) * k.L + col);
 #pragma unroll
       for (int w = 0; w < TW; ++w)
@@ -409,10 +421,10 @@ __device__ __forceinline__ void ket_sum_lanes(const KetArgs& k, int bid, int nbl
   for (size_t t = bid * (size_t)BLOCK + threadIdx.x; t < total; t += (size_t)nblk * BLOCK) {
     const int m = (int)(t / lp);
     const int l = (int)(t - (size_t)m * lp) * LPT;
-    const int e0 = k.multi_pos[m], e1 = k.multi_pos[m + 1];
     Vec<LPT> s;
 #pragma unroll
     for (int q = 0; q < LPT; ++q) s.v[q] = 0.0;
+    const int e0 = k.multi_pos[m], e1 = k.multi_pos[m + 1];
     // rows in groups of 4 loaded together (clamped indices), added in entry order
     for (int e = e0; e < e1; e += 4) {
       Vec<LPT> v[4];
