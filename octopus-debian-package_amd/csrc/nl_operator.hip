@@ -158,19 +158,24 @@ namespace octhip {
 int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 
 // Segment length of a launch with nchunk lane chunks of lc lanes: the operator's setting,
-// or the candidate (48, 32, 28, 24, 20, 16, 12, 8 planes) with the smallest estimated time,
-// in plane steps of one workgroup:
+// or the candidate (48, 32, 28, 24, 22, 21, 20, 16, 12, 8 planes) with the smallest
+// estimated time, in plane steps of one workgroup:
 //   work  = nchunk * sum over items (planes + 8 overlap planes + ZM_PROLOGUE)
-//   time  = max(work / (CUs * g), longest item * r / g)
+//   time  = max(work / (CUs * g), longest item * r / g) * (1 + ZM_TAIL * idle)
 // where r = workgroups resident per CU (2 when the instance's registers allow it and the
 // segment's LDS fits twice, else 1) and g = the throughput of r resident workgroups
 // relative to one (ZM_TWO_GAIN for two).  The first term is the whole launch spread over
 // the chip, the second its critical path: long segments cost little overlap but leave
-// CUs idle on small meshes.  Measured on MI355X: R=12 sphere, 32 complex states: 24 planes
-// with two resident workgroups beat 48 with one by 7%; the 64^3 Laplacian with 8 lanes:
-// 8 planes (24 us against 38 us at 32).
+// CUs idle on small meshes.  idle = the unused fraction of the last wave of workgroups on
+// an XCD (items per XCD over its resident slots, rounded up, minus the unrounded value;
+// counted when there is at least one full wave): near-equal items leave that many slots
+// empty at the end.  Measured on MI355X: R=12 sphere, 32 complex states: 24 planes with
+// two resident workgroups beat 48 with one by 7%, and 21-22 planes (3.7-3.8 waves per
+// XCD) beat 24 (3.4) and 28 (3.1) by 1%; the 64^3 Laplacian with 8 lanes: 8 planes
+// (24 us against 38 us at 32).
 constexpr double ZM_PROLOGUE = 4.0;  // item prologue, in plane steps
 constexpr double ZM_TWO_GAIN = 1.3;  // throughput of two resident workgroups against one
+constexpr double ZM_TAIL = 0.25;     // cost of the idle slots of a partial last wave
 int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident) {
   if (op->zseg > 0) return op->zseg;
   for (const auto& c : op->zseg_cache)
@@ -182,20 +187,24 @@ int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident
   const bool two_ok = resident >= 2;
   double best_t = 1e300;
   int best = 8;
-  for (int zs : {48, 32, 28, 24, 20, 16, 12, 8}) {
+  for (int zs : {48, 32, 28, 24, 22, 21, 20, 16, 12, 8}) {
     const size_t shm = zmarch_shmem(lc, zs);
     if (shm > 160 * 1024) continue;
     const int r = (two_ok && shm <= 80 * 1024) ? 2 : 1;
     const double g = r == 2 ? ZM_TWO_GAIN : 1.0;
     double work = 0.0, cmax = 0.0;
+    long items = 0;
     for (size_t t = 0; t < op->tile_zlo.size(); ++t) {
       if (op->tile_zhi[t] < 0) continue;
       const int len = op->tile_zhi[t] - op->tile_zlo[t] + 1;
       const int n = (len + zs - 1) / zs;
       work += len + n * (8.0 + ZM_PROLOGUE);
       cmax = std::max(cmax, (len + n - 1) / n + 8.0 + ZM_PROLOGUE);
+      items += n;
     }
-    const double tm = std::max(nchunk * work / (ncu * g), cmax * r / g);
+    double tm = std::max(nchunk * work / (ncu * g), cmax * r / g);
+    const double waves = std::ceil((double)items * nchunk / 8.0) / (std::max(ncu / 8, 1) * (double)r);
+    if (waves >= 1.0) tm *= 1.0 + ZM_TAIL * (std::ceil(waves) - waves);
     if (tm < best_t) {
       best_t = tm;
       best = zs;
