@@ -23,7 +23,7 @@
 namespace octhip {
 void h_apply_args(const octhip_hamiltonian_s* h, const double* in, int L, double* ksum, ApplyArgs& a);
 int zmarch_items(octhip_nl_operator_s* op, int zs);
-int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident);
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident, bool alone);
 int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 bool zmarch_fits(const octhip_nl_operator_s* op, const ApplyArgs& a);
 int launch_zmarch_multi(const ZArgs* zs, const int4* sched, int len, int lc, int lpt, int epi, bool nl, int zseg,
@@ -301,7 +301,7 @@ extern "C" int octhip_batched_create(int npairs, const octhip_hamiltonian_t* hms
     if (!rc && taylor_deferred(order) && order == 4) rc = dalloc((void**)&p.y[2], sizeof(double) * h->np * L);
     // one plan per epilogue, segments for the residency of that instance
     for (int plan = 0; plan < 2 && !rc; ++plan) {
-      rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident[plan]));
+      rc = zmarch_items(h->kin, zmarch_zseg(h->kin, p.L / b->lc, b->lc, resident[plan], false));
       if (!rc) {
         p.items[plan] = h->kin->d_items;
         p.tab[plan] = h->kin->d_tab;

@@ -176,10 +176,13 @@ int zmarch_reg_blocks(int lc, int lpt, int epi, bool nl, bool multi);
 constexpr double ZM_PROLOGUE = 4.0;  // item prologue, in plane steps
 constexpr double ZM_TWO_GAIN = 1.3;  // throughput of two resident workgroups against one
 constexpr double ZM_TAIL = 0.25;     // cost of the idle slots of a partial last wave
-int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident) {
+// alone: the launch holds this operator's items only (a batched launch shares its grid
+// with other systems, so the last-wave term of one system does not apply)
+int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident, bool alone) {
   if (op->zseg > 0) return op->zseg;
+  const int rkey = resident | (alone ? 0 : 0x100);
   for (const auto& c : op->zseg_cache)
-    if (c.x == nchunk && c.y == lc && c.z == resident) return c.w;
+    if (c.x == nchunk && c.y == lc && c.z == rkey) return c.w;
   static int ncu = 0;
   if (ncu <= 0 && (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, current_device()) != hipSuccess ||
                    ncu <= 0))
@@ -204,13 +207,13 @@ int zmarch_zseg(const octhip_nl_operator_s* op, int nchunk, int lc, int resident
     }
     double tm = std::max(nchunk * work / (ncu * g), cmax * r / g);
     const double waves = std::ceil((double)items * nchunk / 8.0) / (std::max(ncu / 8, 1) * (double)r);
-    if (waves >= 1.0) tm *= 1.0 + ZM_TAIL * (std::ceil(waves) - waves);
+    if (alone && waves >= 1.0) tm *= 1.0 + ZM_TAIL * (std::ceil(waves) - waves);
     if (tm < best_t) {
       best_t = tm;
       best = zs;
     }
   }
-  op->zseg_cache.push_back(make_int4(nchunk, lc, resident, best));
+  op->zseg_cache.push_back(make_int4(nchunk, lc, rkey, best));
   return best;
 }
 
@@ -230,7 +233,7 @@ void zmarch_shape(const octhip_nl_operator_s* op, int L, int* lc, int* lpt) {
 int zmarch_zseg_for(const octhip_nl_operator_s* op, int L, int epi, bool nl) {
   int lc, lpt;
   zmarch_shape(op, L, &lc, &lpt);
-  return zmarch_zseg(op, L / lc, lc, zmarch_reg_blocks(lc, lpt, epi, nl, false));
+  return zmarch_zseg(op, L / lc, lc, zmarch_reg_blocks(lc, lpt, epi, nl, false), true);
 }
 
 // (tile, z-segment) work items for segment length zs; every list built stays allocated

@@ -127,7 +127,7 @@ struct octhip_nl_operator_s {
   int lane_chunk = 0;                    // lanes per workgroup (0 = automatic)
   int lanes_per_thread = 0;              // doubles per thread (0 = automatic)
   int zseg_built = 0;
-  mutable std::vector<int4> zseg_cache;  // automatic segment lengths: {nchunk, lc, resident, zseg}
+  mutable std::vector<int4> zseg_cache;  // automatic segment lengths: {nchunk, lc, resident (+0x100 in a batched launch), zseg}
   std::vector<int> tile_zlo, tile_zhi;   // per tile: inner plane range (cube z), -1 if empty
   int ntx = 0, nty = 0;
   int4* d_items = nullptr;
