@@ -115,7 +115,8 @@ __global__ __launch_bounds__(BLOCK) void k_bra_plain(const BraArgs b) { bra_plai
 // B[l/16][l%16]; accumulator element r of lane l is D[4*r + l/16][l%16] (checked on the
 // device with an in-bounds probe).
 // A comes from the transposed projector table (16 lanes read consecutive doubles of one
-// sphere point's row), B from the state rows (16 lanes read 128 contiguous bytes).
+// sphere point's row), B from the state rows (16 lanes read 128 contiguous bytes, 256
+// with two columns per lane).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 
@@ -286,8 +287,8 @@ int launch_bra_mfma(const BraArgs& b, const double* pt, int pjs, int nchunk, hip
 // D(16 points x 16 lanes) = A(16 points x 4 projectors) B(4 projectors x 16 lanes); lane
 // (kk, ii) supplies A[ii][kk] = P[q0+ii, jb+kk] and B[kk][ii] = proj[pbase+jb+kk, col];
 // accumulator element r of lane (kk, ii) is D[4r+kk][ii].
-// One workgroup per (chunk, lane tile of 16); the 4 waves take the chunk's groups of 16
-// sphere points in turn.  Loads are unconditional (clamped indices, zero operands).
+// One workgroup per (chunk, lane tile of 16 TW lanes); the 4 waves take the chunk's groups
+// of 16 sphere points in turn.  Loads are unconditional (clamped indices, zero operands).
 // TW = 2: 32-lane tiles, lane ii computes columns 2ii and 2ii+1 with two MFMAs that share
 // the projector operand and stores them as one 16-byte piece (one complex value); each
 // column's sum is the TW = 1 sum, in the same order.
